@@ -1,0 +1,195 @@
+/*
+ * lumina_hip.h -- C ABI of the MI355X (gfx950) CDA solve hot path.
+ *
+ * The entry points replace the reference's device interface for the CDA
+ * solve (Rust crate lumina-compute plus the on-the-fly branch of
+ * lumina-core). Each declaration names the reference item it stands in for
+ * (paths relative to the reference repository root).
+ *
+ * Conventions (mirroring lumina-compute/src/backend.rs):
+ *  - complex numbers are lh_c64 = {re, im}, the layout of
+ *    num_complex::Complex<f64> (#[repr(C)]);
+ *  - vectors are interleaved per dipole, element 3*i + c (cda/mod.rs:397-400);
+ *  - polarisability blocks are row-major [9] (types.rs:12-18);
+ *  - inputs are borrowed; outputs are caller-allocated; the library never
+ *    keeps a caller pointer after a call returns;
+ *  - every call returns an lh_status; lh_last_error() gives the message of
+ *    the last failure on the calling thread;
+ *  - sessions own their device memory and HIP stream; different sessions
+ *    may be used from different threads concurrently (no global lock); a
+ *    single session must not be used by two threads at once.
+ */
+#ifndef LUMINA_HIP_H
+#define LUMINA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* num_complex::Complex<f64> */
+typedef struct {
+  double re, im;
+} lh_c64;
+
+/* ComputeError (backend.rs:13-22) plus SolverError::ConvergenceFailure
+ * (solver/mod.rs:18-31). */
+typedef enum {
+  LH_OK = 0,
+  LH_UNAVAILABLE = 1,          /* ComputeError::Unavailable */
+  LH_DEVICE_ERROR = 2,         /* ComputeError::DeviceError (also dimension mismatch, gpu.rs:411-416) */
+  LH_OUT_OF_MEMORY = 3,        /* ComputeError::OutOfMemory */
+  LH_INVALID_ARGUMENT = 4,
+  LH_CONVERGENCE_FAILURE = 5   /* SolverError::ConvergenceFailure { max_iter, residual } */
+} lh_status;
+
+/* Arithmetic of the far-field pair sum. FP64: everything complex FP64.
+ * MIXED: Arnoldi matvecs evaluate and accumulate the far-field Green's
+ * tensor in FP32 (compensated accumulation); restart residuals, Gram-Schmidt,
+ * Givens rotations, the near-field (FCD) tensors and alpha^-1 stay FP64. */
+typedef enum { LH_PREC_FP64 = 0, LH_PREC_MIXED = 1 } lh_precision;
+
+typedef struct lh_backend lh_backend;
+typedef struct lh_session lh_session;
+typedef struct lh_pairsum lh_pairsum;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lh_last_error(void);
+
+/* DeviceInfo (backend.rs:26-39). backend_type: 0 Cpu, 1 Gpu, 2 Distributed. */
+typedef struct {
+  char name[256];
+  char arch[64];
+  int backend_type;
+  uint64_t memory_bytes;
+  uint32_t compute_units;
+} lh_device_info;
+
+/* GpuBackend::new_blocking (gpu.rs:301) -- one backend per device. */
+int lh_backend_create(int device_ordinal, lh_backend** out);
+void lh_backend_destroy(lh_backend* backend);
+/* ComputeBackend::device_info (backend.rs:64). */
+int lh_backend_device_info(const lh_backend* backend, lh_device_info* out);
+
+/* ---- stored-matrix session: MatvecSession (backend.rs:47-55) ----------- */
+
+/* ComputeBackend::create_session(n) (backend.rs:70-72; gpu.rs:310-399);
+ * n is the number of unknowns (3N). */
+int lh_session_create(lh_backend* backend, size_t n, lh_session** out);
+/* MatvecSession::upload_matrix (backend.rs:51; gpu.rs:401-426); row-major n x n. */
+int lh_session_upload_matrix(lh_session* session, const lh_c64* matrix, size_t n);
+/* MatvecSession::matvec (backend.rs:54; gpu.rs:428-488): y = A x, length n. */
+int lh_session_matvec(lh_session* session, const lh_c64* x, lh_c64* y, size_t n);
+void lh_session_destroy(lh_session* session);
+
+/* ---- on-the-fly pair-sum session --------------------------------------- */
+
+/* The arguments of matvec_on_the_fly (assembly.rs:209-218) with the
+ * Dipole list (types.rs:12-18) split into positions and polarisabilities,
+ * plus the LatticeSpec (lattice.rs:29-44) and SubstrateRuntime::Fresnel
+ * (substrate.rs:103-107) it may carry. */
+typedef struct {
+  size_t n_dipoles;
+  const double* positions;        /* [n][3], nm */
+  const lh_c64* polarisability;   /* [n][9] row-major, or [9] if polarisability_uniform */
+  int polarisability_uniform;
+  double k;                       /* 2 pi n_env / lambda, nm^-1 */
+  int use_fcd;                    /* CdaSolver::use_fcd (cda/mod.rs:53) */
+  double cell_size;               /* CdaSolver::cell_size (cda/mod.rs:55) */
+  int has_lattice;                /* 2D lattice only (a2 present) */
+  double a1[3], a2[3];
+  double ewald_eta;               /* <= 0: automatic sqrt(pi / A) (lattice.rs:96-102) */
+  int truncation_real, truncation_recip;
+  double k_bloch[3];              /* SimulationParams::k_bloch (types.rs:92) */
+  int has_substrate;
+  double z_interface;
+  lh_c64 delta_eps;               /* reflection factor (substrate.rs:191-197) */
+  int precision;                  /* lh_precision */
+} lh_pairsum_spec;
+
+/* Incident plane wave (IncidentField, types.rs:37-74). The polarisation is
+ * complex so that circular polarisation (e.g. (x +- i y)/sqrt 2) can be
+ * expressed; a real polarisation reproduces the reference. */
+typedef struct {
+  double direction[3];
+  lh_c64 polarisation[3];
+  double amplitude;
+} lh_incident;
+
+/* Solve statistics. iterations = total Arnoldi steps (iterative.rs:129);
+ * cycles = restart cycles entered; matvecs = matvecs performed on the device
+ * (the reference's redundant cycle-top matvec, iterative.rs:55, is not
+ * repeated: its result equals the previous cycle's explicit residual);
+ * matvecs_fp32 = how many of those used the FP32 far field;
+ * residual = final ||rhs - A x|| / ||rhs|| (FP64);
+ * pair_evals = Green's pair evaluations performed (N(N-1) per matvec, plus
+ * N^2 image pairs with a substrate). */
+typedef struct {
+  uint64_t iterations;
+  uint64_t cycles;
+  uint64_t matvecs;
+  uint64_t matvecs_fp32;
+  double residual;
+  double pair_evals;
+  int status;                     /* LH_OK or LH_CONVERGENCE_FAILURE */
+} lh_solve_info;
+
+/* Builds the device state for the closure of cda/mod.rs:329-345 / 375-391
+ * (on-the-fly matvec): positions, alpha^-1 blocks (assembly.rs:325-327),
+ * the FCD near-pair list and tensors (greens.rs:94-167). */
+int lh_pairsum_create(lh_backend* backend, const lh_pairsum_spec* spec, lh_pairsum** out);
+/* Re-targets a session to a new wavelength: k, polarisabilities and the
+ * substrate factor change, geometry is kept (runner.rs:150-186). */
+int lh_pairsum_set_wavelength(lh_pairsum* s, double k, const lh_c64* polarisability,
+                              int polarisability_uniform, lh_c64 delta_eps);
+/* matvec_on_the_fly (assembly.rs:209-318): y = A x, host vectors of 3N.
+ * precision selects the far-field arithmetic of this one product. */
+int lh_pairsum_matvec(lh_pairsum* s, const lh_c64* x, lh_c64* y, int precision);
+/* solve_gmres (iterative.rs:33-160) on this session's operator: restarted
+ * GMRES(min(30, 3N)), zero initial guess, stop at ||r|| < tol ||rhs||.
+ * Returns LH_CONVERGENCE_FAILURE (x holds the last iterate) like
+ * SolverError::ConvergenceFailure. */
+int lh_pairsum_solve(lh_pairsum* s, const lh_c64* rhs, double tol, size_t max_iter,
+                     lh_c64* x, lh_solve_info* info);
+/* build_incident_field_vector (assembly.rs:178-194) for this session's dipoles. */
+int lh_pairsum_incident(lh_pairsum* s, const lh_incident* inc, lh_c64* e_inc);
+/* C_ext, C_abs, C_sca of moments p (cda/mod.rs:187-235) -> out[3]. */
+int lh_pairsum_cross_sections(lh_pairsum* s, const lh_c64* p, const lh_c64* e_inc,
+                              double amplitude, double out[3]);
+/* CdaSolver::compute_cross_sections (cda/mod.rs:175-236) on the on-the-fly
+ * path: builds E_inc, solves, reduces. p_out (3N, may be NULL) receives the
+ * moments. */
+int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, double tol,
+                                      size_t max_iter, double out[3], lh_c64* p_out,
+                                      lh_solve_info* info);
+void lh_pairsum_destroy(lh_pairsum* s);
+
+/* ---- batched solves ------------------------------------------------------ */
+
+/* One solve of a batch: a pair-sum spec (its precision field is ignored:
+ * the batch precision applies) and its incident field. Problems whose
+ * positions pointer, n_dipoles, use_fcd and cell_size agree share one
+ * device copy of the geometry and its near-pair list. */
+typedef struct {
+  lh_pairsum_spec spec;
+  lh_incident incident;
+} lh_batch_problem;
+
+/* Many independent compute_cross_sections calls (the rayon wavelength
+ * sweep of runner.rs:145-218 and the GUI sweep, app.rs:297-392) advanced in
+ * lock-step by one stream of batched kernels, the restart cycle captured
+ * once as a HIP graph and replayed. cross_sections: [count][3];
+ * moments (may be NULL): per problem a caller buffer of 3N;
+ * infos (may be NULL): [count]. Returns LH_CONVERGENCE_FAILURE if any
+ * problem failed to converge (its info.status says which). */
+int lh_batch_solve(lh_backend* backend, const lh_batch_problem* problems, size_t count,
+                   double tol, size_t max_iter, int precision, double* cross_sections,
+                   lh_c64* const* moments, lh_solve_info* infos);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUMINA_HIP_H */

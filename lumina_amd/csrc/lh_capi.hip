@@ -1,0 +1,462 @@
+// C ABI (include/lumina_hip.h): backend, stored-matrix session, on-the-fly
+// pair-sum sessions and batched solves.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "lh_engine.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int st, const std::string& m) {
+  g_last_error = m;
+  return st;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    return f();
+  } catch (const lh::Error& e) {
+    return fail(e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(LH_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(LH_DEVICE_ERROR, e.what());
+  }
+}
+
+// assembly.rs:335-363 invert_3x3: cofactor inverse, |det| > 1e-30 required
+// (the reference panics on a singular block; here it is an argument error).
+void invert3(const lh_c64* m, lh_c64* out) {
+  auto mul = [](lh_c64 a, lh_c64 b) {
+    return lh_c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  };
+  auto sub = [](lh_c64 a, lh_c64 b) { return lh_c64{a.re - b.re, a.im - b.im}; };
+  auto add = [](lh_c64 a, lh_c64 b) { return lh_c64{a.re + b.re, a.im + b.im}; };
+  const lh_c64 a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+               k = m[8];
+  const lh_c64 det = add(sub(mul(a, sub(mul(e, k), mul(f, h))), mul(b, sub(mul(d, k), mul(f, g)))),
+                         mul(c, sub(mul(d, h), mul(e, g))));
+  const double dn = std::hypot(det.re, det.im);
+  if (!(dn > 1e-30))
+    throw lh::Error(LH_INVALID_ARGUMENT,
+                    "Singular polarisability tensor (det = " + std::to_string(dn) + ")");
+  const double nsq = det.re * det.re + det.im * det.im;
+  const lh_c64 inv{det.re / nsq, -det.im / nsq};
+  out[0] = mul(inv, sub(mul(e, k), mul(f, h)));
+  out[1] = mul(inv, sub(mul(c, h), mul(b, k)));
+  out[2] = mul(inv, sub(mul(b, f), mul(c, e)));
+  out[3] = mul(inv, sub(mul(f, g), mul(d, k)));
+  out[4] = mul(inv, sub(mul(a, k), mul(c, g)));
+  out[5] = mul(inv, sub(mul(c, d), mul(a, f)));
+  out[6] = mul(inv, sub(mul(d, h), mul(e, g)));
+  out[7] = mul(inv, sub(mul(b, g), mul(a, h)));
+  out[8] = mul(inv, sub(mul(a, e), mul(b, d)));
+}
+
+std::vector<lh_c64> inverse_blocks(const lh_c64* alpha, int uniform, size_t n) {
+  const size_t nb = uniform ? 1 : n;
+  std::vector<lh_c64> out(9 * nb);
+  for (size_t i = 0; i < nb; ++i) invert3(alpha + 9 * i, out.data() + 9 * i);
+  return out;
+}
+
+void check_spec(const lh_pairsum_spec* s) {
+  if (!s) throw lh::Error(LH_INVALID_ARGUMENT, "null spec");
+  if (s->n_dipoles == 0) throw lh::Error(LH_INVALID_ARGUMENT, "No dipoles provided");
+  if (s->n_dipoles > (size_t)(1 << 26)) throw lh::Error(LH_INVALID_ARGUMENT, "too many dipoles");
+  if (!s->positions || !s->polarisability)
+    throw lh::Error(LH_INVALID_ARGUMENT, "null positions or polarisability");
+  if (!(s->k > 0.0) || !std::isfinite(s->k))
+    throw lh::Error(LH_INVALID_ARGUMENT, "k must be positive and finite");
+  if (s->use_fcd && !(s->cell_size > 0.0))
+    throw lh::Error(LH_INVALID_ARGUMENT, "cell_size must be positive with FCD");
+  if (s->has_lattice)
+    throw lh::Error(LH_UNAVAILABLE, "periodic (Ewald) pair sums are not built in this library");
+  if (s->precision != LH_PREC_FP64 && s->precision != LH_PREC_MIXED)
+    throw lh::Error(LH_INVALID_ARGUMENT, "unknown precision");
+}
+
+lh::Geometry make_geometry(const lh_pairsum_spec* s) {
+  lh::Geometry g;
+  g.n = (int)s->n_dipoles;
+  g.use_fcd = s->use_fcd ? 1 : 0;
+  g.cell = s->cell_size;
+  // Far-field exclusion radius: 5% beyond the FCD radius 2 cell_size, so
+  // every FCD pair is in the near list whatever the rounding of |d|.
+  const double R = 2.0 * s->cell_size * 1.05;
+  g.rskip2 = g.use_fcd ? R * R : 0.0;
+  g.x.resize(g.n);
+  g.y.resize(g.n);
+  g.z.resize(g.n);
+  for (int i = 0; i < g.n; ++i) {
+    g.x[i] = s->positions[3 * i];
+    g.y[i] = s->positions[3 * i + 1];
+    g.z[i] = s->positions[3 * i + 2];
+  }
+  lh::build_near_list(g);
+  return g;
+}
+
+lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
+  lh::Problem p;
+  p.geom = geom;
+  p.k = s->k;
+  p.ainv_uniform = s->polarisability_uniform ? 1 : 0;
+  p.ainv = inverse_blocks(s->polarisability, p.ainv_uniform, s->n_dipoles);
+  p.has_sub = s->has_substrate ? 1 : 0;
+  p.z_interface = s->z_interface;
+  p.delta = s->has_substrate ? s->delta_eps : lh_c64{0.0, 0.0};
+  return p;
+}
+
+void fill_info(const lh::SolveResult& r, lh_solve_info* info) {
+  if (!info) return;
+  info->iterations = (uint64_t)r.iterations;
+  info->cycles = (uint64_t)r.cycles;
+  info->matvecs = (uint64_t)r.matvecs;
+  info->matvecs_fp32 = (uint64_t)r.matvecs32;
+  info->residual = r.residual;
+  info->pair_evals = r.pair_evals;
+  info->status = r.status;
+}
+
+}  // namespace
+
+struct lh_backend {
+  int device = 0;
+  hipDeviceProp_t prop;
+};
+
+struct lh_session {
+  lh_backend* b = nullptr;
+  size_t n = 0;
+  hipStream_t stream = nullptr;
+  lh::DevMem A, x, y;
+  bool uploaded = false;
+};
+
+struct lh_pairsum {
+  lh_backend* b = nullptr;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<lh::Engine> eng;
+  size_t n = 0;
+  int precision = 0;
+  lh_c64 delta{0.0, 0.0};
+  int has_sub = 0;
+};
+
+namespace lh {
+void launch_dense_matvec(hipStream_t st, int n, const double2* A, const double2* x, double2* y);
+}
+
+extern "C" {
+
+const char* lh_last_error(void) { return g_last_error.c_str(); }
+
+int lh_backend_create(int device_ordinal, lh_backend** out) {
+  return guarded([&]() {
+    if (!out) return fail(LH_INVALID_ARGUMENT, "null out");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+      (void)hipGetLastError();
+      return fail(LH_UNAVAILABLE, "Backend not available: no HIP device");
+    }
+    if (device_ordinal < 0 || device_ordinal >= count)
+      return fail(LH_INVALID_ARGUMENT, "device ordinal out of range");
+    auto b = std::make_unique<lh_backend>();
+    b->device = device_ordinal;
+    LH_HIP(hipSetDevice(device_ordinal));
+    LH_HIP(hipGetDeviceProperties(&b->prop, device_ordinal));
+    *out = b.release();
+    return (int)LH_OK;
+  });
+}
+
+void lh_backend_destroy(lh_backend* backend) { delete backend; }
+
+int lh_backend_device_info(const lh_backend* b, lh_device_info* out) {
+  return guarded([&]() {
+    if (!b || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, b->prop.name, sizeof(out->name) - 1);
+    std::strncpy(out->arch, b->prop.gcnArchName, sizeof(out->arch) - 1);
+    out->backend_type = 1;
+    out->memory_bytes = (uint64_t)b->prop.totalGlobalMem;
+    out->compute_units = (uint32_t)b->prop.multiProcessorCount;
+    return (int)LH_OK;
+  });
+}
+
+// ---- stored-matrix session ------------------------------------------------
+
+int lh_session_create(lh_backend* b, size_t n, lh_session** out) {
+  return guarded([&]() {
+    if (!b || !out || n == 0) return fail(LH_INVALID_ARGUMENT, "bad argument");
+    *out = nullptr;
+    LH_HIP(hipSetDevice(b->device));
+    auto s = std::make_unique<lh_session>();
+    s->b = b;
+    s->n = n;
+    const size_t bytes = n * n * sizeof(lh_c64);
+    size_t fr = 0, tot = 0;
+    LH_HIP(hipMemGetInfo(&fr, &tot));
+    if (bytes > fr)
+      return fail(LH_OUT_OF_MEMORY, "OutOfMemory: requested " + std::to_string(bytes) +
+                                        " bytes, available " + std::to_string(fr));
+    LH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->A.alloc(bytes);
+    s->x.alloc(n * sizeof(lh_c64));
+    s->y.alloc(n * sizeof(lh_c64));
+    *out = s.release();
+    return (int)LH_OK;
+  });
+}
+
+int lh_session_upload_matrix(lh_session* s, const lh_c64* m, size_t n) {
+  return guarded([&]() {
+    if (!s || !m) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (n != s->n)
+      return fail(LH_DEVICE_ERROR, "Matrix dimension mismatch: session " + std::to_string(s->n) +
+                                       ", matrix " + std::to_string(n));
+    LH_HIP(hipSetDevice(s->b->device));
+    LH_HIP(hipMemcpyAsync(s->A.p, m, n * n * sizeof(lh_c64), hipMemcpyHostToDevice, s->stream));
+    LH_HIP(hipStreamSynchronize(s->stream));
+    s->uploaded = true;
+    return (int)LH_OK;
+  });
+}
+
+int lh_session_matvec(lh_session* s, const lh_c64* x, lh_c64* y, size_t n) {
+  return guarded([&]() {
+    if (!s || !x || !y) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (n != s->n)
+      return fail(LH_DEVICE_ERROR, "Vector dimension mismatch: session " + std::to_string(s->n) +
+                                       ", vector " + std::to_string(n));
+    if (!s->uploaded) return fail(LH_DEVICE_ERROR, "matrix not uploaded");
+    LH_HIP(hipSetDevice(s->b->device));
+    LH_HIP(hipMemcpyAsync(s->x.p, x, n * sizeof(lh_c64), hipMemcpyHostToDevice, s->stream));
+    lh::launch_dense_matvec(s->stream, (int)n, s->A.as<double2>(), s->x.as<double2>(),
+                            s->y.as<double2>());
+    LH_HIP(hipGetLastError());
+    LH_HIP(hipMemcpyAsync(y, s->y.p, n * sizeof(lh_c64), hipMemcpyDeviceToHost, s->stream));
+    LH_HIP(hipStreamSynchronize(s->stream));
+    return (int)LH_OK;
+  });
+}
+
+void lh_session_destroy(lh_session* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->b->device);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// ---- pair-sum session -----------------------------------------------------
+
+int lh_pairsum_create(lh_backend* b, const lh_pairsum_spec* spec, lh_pairsum** out) {
+  return guarded([&]() {
+    if (!b || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    check_spec(spec);
+    LH_HIP(hipSetDevice(b->device));
+    auto s = std::make_unique<lh_pairsum>();
+    s->b = b;
+    s->n = spec->n_dipoles;
+    s->precision = spec->precision;
+    s->has_sub = spec->has_substrate ? 1 : 0;
+    s->delta = spec->delta_eps;
+    LH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    std::vector<lh::Geometry> geoms;
+    geoms.push_back(make_geometry(spec));
+    std::vector<lh::Problem> probs;
+    probs.push_back(make_problem(spec, 0));
+    s->eng = std::make_unique<lh::Engine>(s->stream);
+    lh::Tiling t;
+    s->eng->build(geoms, probs, t);
+    *out = s.release();
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_set_wavelength(lh_pairsum* s, double k, const lh_c64* alpha, int uniform,
+                              lh_c64 delta_eps) {
+  return guarded([&]() {
+    if (!s || !alpha) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(k > 0.0) || !std::isfinite(k)) return fail(LH_INVALID_ARGUMENT, "k must be positive");
+    LH_HIP(hipSetDevice(s->b->device));
+    lh::Problem p;
+    p.k = k;
+    p.ainv_uniform = uniform ? 1 : 0;
+    p.ainv = inverse_blocks(alpha, p.ainv_uniform, s->n);
+    p.delta = s->has_sub ? delta_eps : lh_c64{0.0, 0.0};
+    s->eng->set_problem(0, p);
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_matvec(lh_pairsum* s, const lh_c64* x, lh_c64* y, int precision) {
+  return guarded([&]() {
+    if (!s || !x || !y) return fail(LH_INVALID_ARGUMENT, "null argument");
+    LH_HIP(hipSetDevice(s->b->device));
+    s->eng->upload_slot(0, 0, x);
+    s->eng->matvec_all(0, precision == LH_PREC_MIXED);
+    s->eng->download_slot(0, LH_SLOT_W, y);
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_solve(lh_pairsum* s, const lh_c64* rhs, double tol, size_t max_iter, lh_c64* x,
+                     lh_solve_info* info) {
+  return guarded([&]() {
+    if (!s || !rhs || !x) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
+    LH_HIP(hipSetDevice(s->b->device));
+    s->eng->upload_slot(0, LH_SLOT_RHS, rhs);
+    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, false);
+    s->eng->download_slot(0, LH_SLOT_X, x);
+    const auto r = s->eng->results();
+    fill_info(r[0], info);
+    if (r[0].status != LH_OK)
+      return fail(LH_CONVERGENCE_FAILURE,
+                  "GMRES did not converge within " + std::to_string(max_iter) +
+                      " iterations (residual " + std::to_string(r[0].residual) + ")");
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_incident(lh_pairsum* s, const lh_incident* inc, lh_c64* e_inc) {
+  return guarded([&]() {
+    if (!s || !inc || !e_inc) return fail(LH_INVALID_ARGUMENT, "null argument");
+    LH_HIP(hipSetDevice(s->b->device));
+    s->eng->fill_incident(0, *inc);
+    s->eng->download_slot(0, LH_SLOT_E, e_inc);
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_cross_sections(lh_pairsum* s, const lh_c64* p, const lh_c64* e_inc,
+                              double amplitude, double out[3]) {
+  return guarded([&]() {
+    if (!s || !p || !e_inc || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+    LH_HIP(hipSetDevice(s->b->device));
+    s->eng->upload_slot(0, LH_SLOT_X, p);
+    s->eng->upload_slot(0, LH_SLOT_E, e_inc);
+    const auto cs = s->eng->cross_sections({amplitude});
+    out[0] = cs[0];
+    out[1] = cs[1];
+    out[2] = cs[2];
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, double tol,
+                                      size_t max_iter, double out[3], lh_c64* p_out,
+                                      lh_solve_info* info) {
+  return guarded([&]() {
+    if (!s || !inc || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+    LH_HIP(hipSetDevice(s->b->device));
+    s->eng->fill_incident(0, *inc);
+    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, false);
+    const auto r = s->eng->results();
+    fill_info(r[0], info);
+    const auto cs = s->eng->cross_sections({inc->amplitude});
+    out[0] = cs[0];
+    out[1] = cs[1];
+    out[2] = cs[2];
+    if (p_out) s->eng->download_slot(0, LH_SLOT_X, p_out);
+    if (r[0].status != LH_OK)
+      return fail(LH_CONVERGENCE_FAILURE,
+                  "GMRES did not converge within " + std::to_string(max_iter) +
+                      " iterations (residual " + std::to_string(r[0].residual) + ")");
+    return (int)LH_OK;
+  });
+}
+
+void lh_pairsum_destroy(lh_pairsum* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->b->device);
+  s->eng.reset();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// ---- batch ----------------------------------------------------------------
+
+int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count, double tol,
+                   size_t max_iter, int precision, double* cross_sections,
+                   lh_c64* const* moments, lh_solve_info* infos) {
+  return guarded([&]() {
+    if (!b || !problems || count == 0 || !cross_sections)
+      return fail(LH_INVALID_ARGUMENT, "bad argument");
+    if (precision != LH_PREC_FP64 && precision != LH_PREC_MIXED)
+      return fail(LH_INVALID_ARGUMENT, "unknown precision");
+    LH_HIP(hipSetDevice(b->device));
+    struct Key {
+      const double* p;
+      size_t n;
+      int fcd;
+      double cell;
+    };
+    std::vector<Key> keys;
+    std::vector<lh::Geometry> geoms;
+    std::vector<lh::Problem> probs;
+    std::vector<double> amp;
+    for (size_t q = 0; q < count; ++q) {
+      lh_pairsum_spec sp = problems[q].spec;
+      sp.precision = precision;
+      check_spec(&sp);
+      int gi = -1;
+      for (size_t t = 0; t < keys.size(); ++t)
+        if (keys[t].p == sp.positions && keys[t].n == sp.n_dipoles &&
+            keys[t].fcd == (sp.use_fcd ? 1 : 0) && keys[t].cell == sp.cell_size) {
+          gi = (int)t;
+          break;
+        }
+      if (gi < 0) {
+        gi = (int)geoms.size();
+        keys.push_back(Key{sp.positions, sp.n_dipoles, sp.use_fcd ? 1 : 0, sp.cell_size});
+        geoms.push_back(make_geometry(&sp));
+      }
+      probs.push_back(make_problem(&sp, gi));
+      amp.push_back(problems[q].incident.amplitude);
+      if (!(problems[q].incident.amplitude != 0.0))
+        return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+    }
+    hipStream_t st;
+    LH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    lh::Engine eng(st);
+    lh::Tiling t;
+    eng.build(geoms, probs, t);
+    for (size_t q = 0; q < count; ++q) eng.fill_incident((int)q, problems[q].incident);
+    eng.solve(tol, (long long)max_iter, precision == LH_PREC_MIXED, true);
+    const auto res = eng.results();
+    const auto cs = eng.cross_sections(amp);
+    std::memcpy(cross_sections, cs.data(), sizeof(double) * cs.size());
+    bool all_ok = true;
+    for (size_t q = 0; q < count; ++q) {
+      if (infos) fill_info(res[q], &infos[q]);
+      if (moments && moments[q]) eng.download_slot((int)q, LH_SLOT_X, moments[q]);
+      if (res[q].status != LH_OK) all_ok = false;
+    }
+    if (!all_ok) return fail(LH_CONVERGENCE_FAILURE, "at least one solve did not converge");
+    return (int)LH_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Device helpers shared by the pair-sum and Krylov kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "lh_types.h"
+
+namespace lh {
+
+// |d|^2 without fused multiply-add. Every kernel (and the host near-list
+// builder) that decides whether a pair belongs to the near list evaluates
+// exactly this expression, so the far-field exclusion and the near list can
+// never disagree.
+__host__ __device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  double a = dx * dx;
+  double b = dy * dy;
+  double c = dz * dz;
+  return (a + b) + c;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of v over a block of LH_VB threads; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds /* >= 4 */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return s;
+}
+
+// Point dyadic Green's tensor G(d) = pf [a I + b d d^T / r^2] with
+// pf = k^2 e^{ikr} / (4 pi r), a = 1 + (ikr - 1)/(kr)^2,
+// b = (3 - 3ikr - (kr)^2)/(kr)^2 (greens.rs:34-71), returned as
+// G = A I + B d d^T. c0 = k^2 / (4 pi).
+__device__ __forceinline__ void g_coeffs64(double r2, double k, double inv_k, double c0,
+                                          double& Ar, double& Ai, double& Br, double& Bi) {
+  const double inv_r = rsqrt(r2);
+  const double r = r2 * inv_r;
+  double s, c;
+  sincos(k * r, &s, &c);
+  const double u = inv_r * inv_k;  // 1 / (k r)
+  const double u2 = u * u;
+  const double pf = c0 * inv_r;
+  const double pr = pf * c, pi = pf * s;
+  const double ar = 1.0 - u2, ai = u;
+  const double br = 3.0 * u2 - 1.0, bi = -3.0 * u;
+  const double ir2 = inv_r * inv_r;
+  Ar = pr * ar - pi * ai;
+  Ai = pr * ai + pi * ar;
+  Br = (pr * br - pi * bi) * ir2;
+  Bi = (pr * bi + pi * br) * ir2;
+}
+
+// The six independent entries (xx yy zz xy xz yz) of the point tensor.
+__device__ __forceinline__ void g6_point64(double dx, double dy, double dz, double k,
+                                           double inv_k, double c0, double t[12]) {
+  double Ar, Ai, Br, Bi;
+  g_coeffs64(r2_exact(dx, dy, dz), k, inv_k, c0, Ar, Ai, Br, Bi);
+  const double d[3] = {dx, dy, dz};
+  t[0] = Ar + Br * dx * dx; t[1] = Ai + Bi * dx * dx;
+  t[2] = Ar + Br * dy * dy; t[3] = Ai + Bi * dy * dy;
+  t[4] = Ar + Br * dz * dz; t[5] = Ai + Bi * dz * dz;
+  t[6] = Br * dx * dy;      t[7] = Bi * dx * dy;
+  t[8] = Br * dx * dz;      t[9] = Bi * dx * dz;
+  t[10] = Br * dy * dz;     t[11] = Bi * dy * dz;
+  (void)d;
+}
+
+}  // namespace lh

@@ -1,0 +1,122 @@
+// Host side of the batched CDA solve engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "lh_types.h"
+#include "../../include/lumina_hip.h"
+
+namespace lh {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int st, const std::string& m) : std::runtime_error(m), status(st) {}
+};
+
+void check_hip(hipError_t e, const char* what);
+#define LH_HIP(x) ::lh::check_hip((x), #x)
+
+// Device allocation owned by one engine; freed with it.
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem();
+  void alloc(size_t b);
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// One dipole geometry (positions + FCD near-pair list), shared by every
+// solve on it.
+struct Geometry {
+  int n = 0;
+  int use_fcd = 0;
+  double cell = 0.0;
+  double rskip2 = 0.0;
+  std::vector<double> x, y, z;
+  std::vector<long long> rp;   // n+1, local entry ids
+  std::vector<int> row, col;
+  std::vector<unsigned char> fcd;
+};
+
+// Builds the near list: every pair j != i with |r_i - r_j|^2 <= rskip2,
+// flagged FCD when sqrt(|r_i - r_j|^2) <= 2 cell_size (greens.rs:106).
+void build_near_list(Geometry& g);
+
+struct Problem {
+  int geom = 0;
+  double k = 0.0;
+  std::vector<lh_c64> ainv;     // 9 (uniform) or 9n
+  int ainv_uniform = 1;
+  int has_sub = 0;
+  double z_interface = 0.0;
+  lh_c64 delta{0.0, 0.0};
+};
+
+// Far-field tile choice (the autotuner's search space).
+struct Tiling {
+  int target_blocks = 4096;   // desired far-field work items per matvec
+  int min_chunk = 128;        // shortest source chunk
+};
+
+struct SolveResult {
+  int status = 0;
+  int iterations = 0, cycles = 0, matvecs = 0, matvecs32 = 0;
+  double residual = 0.0;
+  double pair_evals = 0.0;
+};
+
+class Engine {
+ public:
+  Engine(hipStream_t stream) : stream_(stream) {}
+  ~Engine();
+  void build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t);
+  // Re-targets problem s (same geometry): k, alpha^-1, substrate factor.
+  void set_problem(int s, const Problem& p);
+  void compute_near_tensors();
+  int num_solves() const { return (int)probs_.size(); }
+  int n_of(int s) const { return geoms_[probs_[s].geom].n; }
+
+  // Host <-> slot copies (3n complex).
+  void upload_slot(int s, int slot, const lh_c64* h);
+  void download_slot(int s, int slot, lh_c64* h);
+  void fill_incident(int s, const lh_incident& inc);   // RHS and E slots
+
+  // One y = A x for every solve: x from `in_slot`, y to W.
+  void matvec_all(int in_slot, bool fp32);
+  // GMRES on every solve, RHS slot -> X slot.
+  void solve(double tol, long long max_iter, bool mixed, bool use_graph);
+  std::vector<SolveResult> results();
+  // C_ext, C_abs, C_sca per solve from the X (moments) and E slots.
+  std::vector<double> cross_sections(const std::vector<double>& amplitude);
+
+  double pairs_per_matvec(int s) const;
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
+  void arnoldi_step(int j, bool fp32);
+  int read_counter();
+  void upload_solves();
+
+  hipStream_t stream_;
+  std::vector<Geometry> geoms_;
+  std::vector<Problem> probs_;
+  std::vector<SolveDev> sd_;
+  std::vector<FarItem> far_;
+  std::vector<VecItem> vit_;
+  bool any_sub_ = false, any_free_ = false;
+  long long t_stride_ = 0;
+  int n_titems_ = 0;
+
+  DevMem d_px_, d_py_, d_pz_, d_rp_, d_row_, d_col_, d_fcd_, d_t_, d_ainv_, d_vec_, d_src64_,
+      d_src32_, d_part_, d_dot_, d_norm_, d_cross_, d_solves_, d_state_, d_counter_, d_far_,
+      d_vit_, d_tis_, d_tie_, d_tee_;
+  hipGraphExec_t step_graph_[LH_MMAX][2] = {};
+  bool graph_ready_[LH_MMAX][2] = {};
+};
+
+}  // namespace lh

@@ -1,0 +1,533 @@
+// Host side of the batched CDA solve engine: batch layout, near lists, the
+// GMRES cycle driver and the cross-section reduction.
+#include "lh_engine.h"
+#include "lh_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace lh {
+
+// kernel launchers (lh_pairsum.hip / lh_krylov.hip)
+void launch_near_tensors(hipStream_t, int, const SolveDev*, const int*, const long long*,
+                         const long long*, const double*, const double*, const double*,
+                         const int*, const int*, const unsigned char*, double*, long long);
+void launch_prep64(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
+                   const double*, const double*, const double*, const double2*, int, Src64*);
+void launch_prep32(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
+                   const double*, const double*, const double*, const double2*, int, Src32*);
+void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
+                  const double*, const double*, const double*, const Src64*, double2*);
+void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
+                  const double*, const double*, const double*, const Src32*, double2*);
+void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
+                   const long long*, const int*, const double*, long long, const double2*,
+                   double2*, int, int, const double2*, int, double*);
+void launch_vnorm(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
+                  const double2*, int, double*);
+void launch_scale(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
+                  double2*, int, int);
+void launch_dots(hipStream_t, int, const VecItem*, const SolveDev*, const GState*,
+                 const double2*, int, double2*);
+void launch_update(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, double2*,
+                   int, double*);
+void launch_xupdate(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, double2*);
+void launch_cross(hipStream_t, int, const VecItem*, const SolveDev*, const double2*,
+                  const double2*, double*);
+void launch_copy_slot(hipStream_t, int, const VecItem*, const SolveDev*, double2*, int, int);
+void launch_zero(hipStream_t, int*);
+void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, const double2*, int);
+void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const double*, int, int*);
+void launch_backsub(hipStream_t, int, GState*);
+void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*);
+
+void check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int st = (e == hipErrorOutOfMemory) ? LH_OUT_OF_MEMORY : LH_DEVICE_ERROR;
+  throw Error(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevMem::~DevMem() {
+  if (p) (void)hipFree(p);
+}
+
+void DevMem::alloc(size_t b) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  bytes = b;
+  if (b == 0) return;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    throw Error(LH_OUT_OF_MEMORY, "OutOfMemory: requested " + std::to_string(b) +
+                                      " bytes, available " + std::to_string(fr));
+  }
+}
+
+template <class T>
+static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
+  m.alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+  if (!v.empty())
+    LH_HIP(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// Near list: spatial hash with bins of side sqrt(rskip2); every pair within
+// the exclusion radius is found by scanning the 27 neighbouring bins.
+// ---------------------------------------------------------------------------
+void build_near_list(Geometry& g) {
+  const int n = g.n;
+  g.rp.assign(n + 1, 0);
+  g.row.clear();
+  g.col.clear();
+  g.fcd.clear();
+  if (!g.use_fcd || n == 0) return;
+  const double h = std::sqrt(g.rskip2);
+  double mn[3] = {g.x[0], g.y[0], g.z[0]};
+  for (int i = 0; i < n; ++i) {
+    mn[0] = std::min(mn[0], g.x[i]);
+    mn[1] = std::min(mn[1], g.y[i]);
+    mn[2] = std::min(mn[2], g.z[i]);
+  }
+  auto key = [](long long a, long long b, long long c) {
+    return (a * 73856093LL) ^ (b * 19349663LL) ^ (c * 83492791LL);
+  };
+  std::vector<long long> bx(n), by(n), bz(n);
+  std::unordered_map<long long, std::vector<int>> bins;
+  bins.reserve(n);
+  for (int i = 0; i < n; ++i) {
+    bx[i] = (long long)std::floor((g.x[i] - mn[0]) / h);
+    by[i] = (long long)std::floor((g.y[i] - mn[1]) / h);
+    bz[i] = (long long)std::floor((g.z[i] - mn[2]) / h);
+    bins[key(bx[i], by[i], bz[i])].push_back(i);
+  }
+  std::vector<int> cand;
+  for (int i = 0; i < n; ++i) {
+    cand.clear();
+    for (int a = -1; a <= 1; ++a)
+      for (int b = -1; b <= 1; ++b)
+        for (int c = -1; c <= 1; ++c) {
+          auto it = bins.find(key(bx[i] + a, by[i] + b, bz[i] + c));
+          if (it == bins.end()) continue;
+          for (int j : it->second) {
+            if (j == i) continue;
+            if (bx[j] != bx[i] + a || by[j] != by[i] + b || bz[j] != bz[i] + c) continue;
+            const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
+            if (r2_exact(dx, dy, dz) <= g.rskip2) cand.push_back(j);
+          }
+        }
+    std::sort(cand.begin(), cand.end());
+    for (int j : cand) {
+      const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
+      // greens.rs:102-106: dist = sqrt(dx^2 + dy^2 + dz^2); FCD unless dist > 2 cell
+      const double dist = std::sqrt(r2_exact(dx, dy, dz));
+      g.row.push_back(i);
+      g.col.push_back(j);
+      g.fcd.push_back((dist > 2.0 * g.cell) ? 0 : 1);
+    }
+    g.rp[i + 1] = (long long)g.col.size();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batch construction.
+// ---------------------------------------------------------------------------
+
+Engine::~Engine() {
+  for (int j = 0; j < LH_MMAX; ++j)
+    for (int f = 0; f < 2; ++f)
+      if (graph_ready_[j][f]) (void)hipGraphExecDestroy(step_graph_[j][f]);
+}
+
+static int far_chunks(int n, int chunk) { return (n + chunk - 1) / chunk; }
+
+void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t) {
+  geoms_ = std::move(geoms);
+  probs_ = std::move(probs);
+  const int S = (int)probs_.size();
+  if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
+
+  // geometry arrays
+  std::vector<double> px, py, pz;
+  std::vector<long long> rp;
+  std::vector<int> row, col;
+  std::vector<unsigned char> fcd;
+  std::vector<int> g_pos(geoms_.size()), g_rp(geoms_.size());
+  std::vector<long long> g_e0(geoms_.size());
+  for (size_t gi = 0; gi < geoms_.size(); ++gi) {
+    Geometry& g = geoms_[gi];
+    g_pos[gi] = (int)px.size();
+    g_rp[gi] = (int)rp.size();
+    const long long base = (long long)col.size();
+    g_e0[gi] = base;
+    px.insert(px.end(), g.x.begin(), g.x.end());
+    py.insert(py.end(), g.y.begin(), g.y.end());
+    pz.insert(pz.end(), g.z.begin(), g.z.end());
+    for (long long v : g.rp) rp.push_back(base + v);
+    row.insert(row.end(), g.row.begin(), g.row.end());
+    col.insert(col.end(), g.col.begin(), g.col.end());
+    fcd.insert(fcd.end(), g.fcd.begin(), g.fcd.end());
+  }
+
+  // far-field chunk length: the shortest multiple of 64 (>= min_chunk) whose
+  // item count does not exceed the target
+  long long maxn = 0;
+  for (auto& p : probs_) maxn = std::max<long long>(maxn, geoms_[p.geom].n);
+  int chunk = (int)std::max<long long>(64, ((maxn + 63) / 64) * 64);
+  for (int c = std::max(64, t.min_chunk); c < chunk; c += 64) {
+    long long items = 0;
+    for (auto& p : probs_) {
+      const int n = geoms_[p.geom].n;
+      items += (long long)((n + 255) / 256) * far_chunks(n, c);
+    }
+    if (items <= t.target_blocks) {
+      chunk = c;
+      break;
+    }
+  }
+
+  sd_.assign(S, SolveDev{});
+  far_.clear();
+  vit_.clear();
+  long long vec_off = 0, src_off = 0, part_off = 0, t_off = 0, ainv_off = 0;
+  int vi_off = 0;
+  std::vector<lh_c64> ainv_all;
+  std::vector<int> tis;
+  std::vector<long long> tie, tee;
+  any_sub_ = any_free_ = false;
+  for (int s = 0; s < S; ++s) {
+    const Problem& p = probs_[s];
+    const Geometry& g = geoms_[p.geom];
+    SolveDev& d = sd_[s];
+    d.n = g.n;
+    d.m = std::min(LH_MMAX, 3 * g.n);
+    d.pos_off = g_pos[p.geom];
+    d.rp_off = g_rp[p.geom];
+    d.e0 = g_e0[p.geom];
+    d.t_off = t_off;
+    d.ainv_off = ainv_off;
+    d.ainv_uniform = p.ainv_uniform;
+    d.has_sub = p.has_sub;
+    d.vec_off = vec_off;
+    d.src_off = src_off;
+    d.part_off = part_off;
+    d.nsc = far_chunks(g.n, chunk);
+    d.vi_off = vi_off;
+    d.nvi = (g.n + LH_VB - 1) / LH_VB;
+    d.max_iter = 0;
+    d.k = p.k;
+    d.inv_k = 1.0 / p.k;
+    d.c0 = p.k * p.k / (4.0 * M_PI);
+    d.z2 = 2.0 * p.z_interface;
+    d.dre = p.delta.re;
+    d.dim = p.delta.im;
+    d.rskip2 = g.rskip2;
+    d.cell = g.cell;
+    d.tol = 0.0;
+    if (p.has_sub) any_sub_ = true;
+    else any_free_ = true;
+    for (int t0 = 0; t0 < g.n; t0 += 256)
+      for (int c = 0; c < d.nsc; ++c) {
+        FarItem f;
+        f.s = s;
+        f.t0 = t0;
+        f.c0 = c * chunk;
+        f.c1 = std::min(g.n, (c + 1) * chunk);
+        f.chunk = c;
+        f.pad = 0;
+        far_.push_back(f);
+      }
+    for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
+    const long long nnz = (long long)g.col.size();
+    for (long long e = 0; e < nnz; e += 256) {
+      tis.push_back(s);
+      tie.push_back(d.e0 + e);
+      tee.push_back(d.e0 + std::min(nnz, e + 256));
+    }
+    ainv_all.insert(ainv_all.end(), p.ainv.begin(), p.ainv.end());
+    ainv_off += (long long)p.ainv.size();
+    t_off += nnz;
+    vec_off += (long long)LH_NSLOT * 3 * g.n;
+    src_off += g.n;
+    part_off += (long long)d.nsc * 3 * g.n;
+    vi_off += d.nvi;
+  }
+  t_stride_ = std::max<long long>(t_off, 1);
+  n_titems_ = (int)tis.size();
+
+  hipStream_t st = stream_;
+  upload(d_px_, px, st);
+  upload(d_py_, py, st);
+  upload(d_pz_, pz, st);
+  upload(d_rp_, rp, st);
+  upload(d_row_, row, st);
+  upload(d_col_, col, st);
+  upload(d_fcd_, fcd, st);
+  upload(d_ainv_, ainv_all, st);
+  upload(d_far_, far_, st);
+  upload(d_vit_, vit_, st);
+  upload(d_tis_, tis, st);
+  upload(d_tie_, tie, st);
+  upload(d_tee_, tee, st);
+  d_t_.alloc(sizeof(double) * 12 * (size_t)t_stride_);
+  d_vec_.alloc(sizeof(double2) * (size_t)std::max<long long>(vec_off, 1));
+  LH_HIP(hipMemsetAsync(d_vec_.p, 0, d_vec_.bytes, st));
+  d_src64_.alloc(sizeof(Src64) * (size_t)src_off);
+  d_src32_.alloc(sizeof(Src32) * (size_t)src_off);
+  d_part_.alloc(sizeof(double2) * (size_t)std::max<long long>(part_off, 1));
+  d_dot_.alloc(sizeof(double2) * (size_t)vi_off * LH_NDOT);
+  d_norm_.alloc(sizeof(double) * (size_t)vi_off);
+  d_cross_.alloc(sizeof(double) * 3 * (size_t)vi_off);
+  d_state_.alloc(sizeof(GState) * (size_t)S);
+  LH_HIP(hipMemsetAsync(d_state_.p, 0, d_state_.bytes, st));
+  d_counter_.alloc(sizeof(int) * 4);
+  upload_solves();
+  compute_near_tensors();
+  LH_HIP(hipStreamSynchronize(st));
+}
+
+void Engine::upload_solves() {
+  upload(d_solves_, sd_, stream_);
+}
+
+void Engine::set_problem(int s, const Problem& p) {
+  if (s < 0 || s >= (int)probs_.size()) throw Error(LH_INVALID_ARGUMENT, "bad solve index");
+  Problem& q = probs_[s];
+  const Geometry& g = geoms_[q.geom];
+  if ((int)p.ainv.size() != (p.ainv_uniform ? 9 : 9 * g.n) ||
+      p.ainv_uniform != q.ainv_uniform)
+    throw Error(LH_INVALID_ARGUMENT, "polarisability layout differs from the session's");
+  q.k = p.k;
+  q.ainv = p.ainv;
+  q.delta = p.delta;
+  SolveDev& d = sd_[s];
+  d.k = p.k;
+  d.inv_k = 1.0 / p.k;
+  d.c0 = p.k * p.k / (4.0 * M_PI);
+  d.dre = p.delta.re;
+  d.dim = p.delta.im;
+  LH_HIP(hipMemcpyAsync(d_ainv_.as<double2>() + d.ainv_off, q.ainv.data(),
+                        q.ainv.size() * sizeof(lh_c64), hipMemcpyHostToDevice, stream_));
+  LH_HIP(hipMemcpyAsync(d_solves_.as<SolveDev>() + s, &d, sizeof(SolveDev),
+                        hipMemcpyHostToDevice, stream_));
+  compute_near_tensors();
+  LH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::compute_near_tensors() {
+  launch_near_tensors(stream_, n_titems_, d_solves_.as<SolveDev>(), d_tis_.as<int>(),
+                      d_tie_.as<long long>(), d_tee_.as<long long>(), d_px_.as<double>(),
+                      d_py_.as<double>(), d_pz_.as<double>(), d_row_.as<int>(),
+                      d_col_.as<int>(), d_fcd_.as<unsigned char>(), d_t_.as<double>(),
+                      t_stride_);
+  LH_HIP(hipGetLastError());
+}
+
+void Engine::upload_slot(int s, int slot, const lh_c64* h) {
+  const SolveDev& d = sd_[s];
+  LH_HIP(hipMemcpyAsync(d_vec_.as<double2>() + d.vec_off + (long long)slot * 3 * d.n, h,
+                        sizeof(lh_c64) * 3 * d.n, hipMemcpyHostToDevice, stream_));
+}
+
+void Engine::download_slot(int s, int slot, lh_c64* h) {
+  const SolveDev& d = sd_[s];
+  LH_HIP(hipMemcpyAsync(h, d_vec_.as<double2>() + d.vec_off + (long long)slot * 3 * d.n,
+                        sizeof(lh_c64) * 3 * d.n, hipMemcpyDeviceToHost, stream_));
+  LH_HIP(hipStreamSynchronize(stream_));
+}
+
+// types.rs:60-74 IncidentField::at_position, E0 e exp(i k d.r), for every
+// dipole; written to the RHS and E slots.
+void Engine::fill_incident(int s, const lh_incident& inc) {
+  const Problem& p = probs_[s];
+  const Geometry& g = geoms_[p.geom];
+  std::vector<lh_c64> e(3 * (size_t)g.n);
+  for (int i = 0; i < g.n; ++i) {
+    const double kdotr =
+        p.k * (inc.direction[0] * g.x[i] + inc.direction[1] * g.y[i] + inc.direction[2] * g.z[i]);
+    const double pr = std::cos(kdotr), pi = std::sin(kdotr);
+    for (int c = 0; c < 3; ++c) {
+      const double er = inc.amplitude * inc.polarisation[c].re;
+      const double ei = inc.amplitude * inc.polarisation[c].im;
+      e[3 * i + c] = lh_c64{er * pr - ei * pi, er * pi + ei * pr};
+    }
+  }
+  upload_slot(s, LH_SLOT_RHS, e.data());
+  upload_slot(s, LH_SLOT_E, e.data());
+  LH_HIP(hipStreamSynchronize(stream_));
+}
+
+double Engine::pairs_per_matvec(int s) const {
+  const double n = geoms_[probs_[s].geom].n;
+  return n * (n - 1.0) + (probs_[s].has_sub ? n * n : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Matvec and GMRES cycle.
+// ---------------------------------------------------------------------------
+
+void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
+  hipStream_t st = stream_;
+  const int nv = (int)vit_.size(), nf = (int)far_.size();
+  const VecItem* vi = d_vit_.as<VecItem>();
+  const SolveDev* sd = d_solves_.as<SolveDev>();
+  const GState* gs = d_state_.as<GState>();
+  double2* vec = d_vec_.as<double2>();
+  if (fp32) {
+    launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
+                  d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
+    if (any_free_)
+      launch_far32(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
+                   d_py_.as<double>(), d_pz_.as<double>(), d_src32_.as<Src32>(),
+                   d_part_.as<double2>());
+    if (any_sub_)
+      launch_far32(st, true, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
+                   d_py_.as<double>(), d_pz_.as<double>(), d_src32_.as<Src32>(),
+                   d_part_.as<double2>());
+  } else {
+    launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
+                  d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
+    if (any_free_)
+      launch_far64(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
+                   d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
+                   d_part_.as<double2>());
+    if (any_sub_)
+      launch_far64(st, true, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
+                   d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
+                   d_part_.as<double2>());
+  }
+  launch_reduce(st, nv, vi, sd, gs, which, d_rp_.as<long long>(), d_col_.as<int>(),
+                d_t_.as<double>(), t_stride_, d_ainv_.as<double2>(), vec, in_slot, out_slot,
+                d_part_.as<double2>(), mode, d_norm_.as<double>());
+}
+
+void Engine::matvec_all(int in_slot, bool fp32) {
+  matvec(3, in_slot, LH_SLOT_W, 0, fp32);
+  LH_HIP(hipGetLastError());
+  LH_HIP(hipStreamSynchronize(stream_));
+}
+
+int Engine::read_counter() {
+  int c = 0;
+  LH_HIP(hipMemcpyAsync(&c, d_counter_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  LH_HIP(hipStreamSynchronize(stream_));
+  return c;
+}
+
+void Engine::arnoldi_step(int j, bool fp32) {
+  hipStream_t st = stream_;
+  const int nv = (int)vit_.size(), S = (int)sd_.size();
+  const VecItem* vi = d_vit_.as<VecItem>();
+  const SolveDev* sd = d_solves_.as<SolveDev>();
+  GState* gs = d_state_.as<GState>();
+  double2* vec = d_vec_.as<double2>();
+  matvec(0, j, LH_SLOT_W, 0, fp32);
+  for (int pass = 0; pass < 2; ++pass) {
+    launch_dots(st, nv, vi, sd, gs, vec, j, d_dot_.as<double2>());
+    launch_fin_dots(st, S, sd, gs, j, d_dot_.as<double2>(), pass);
+    launch_update(st, nv, vi, sd, gs, vec, j, d_norm_.as<double>());
+  }
+  launch_zero(st, d_counter_.as<int>());
+  launch_fin_step(st, S, sd, gs, j, d_norm_.as<double>(), fp32 ? 1 : 0, d_counter_.as<int>());
+  launch_scale(st, nv, vi, sd, gs, 2, vec, LH_SLOT_W, -1);
+}
+
+void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
+  (void)use_graph;
+  hipStream_t st = stream_;
+  const int nv = (int)vit_.size(), S = (int)sd_.size();
+  const VecItem* vi = d_vit_.as<VecItem>();
+  SolveDev* sdd = d_solves_.as<SolveDev>();
+  GState* gs = d_state_.as<GState>();
+  double2* vec = d_vec_.as<double2>();
+  for (auto& d : sd_) {
+    d.tol = tol;
+    d.max_iter = (int)std::min<long long>(max_iter, 1LL << 30);
+  }
+  upload_solves();
+  sdd = d_solves_.as<SolveDev>();
+  // x = 0, r = rhs - A 0 = rhs (the matvec of the zero vector is exactly 0)
+  launch_copy_slot(st, nv, vi, sdd, vec, -1, LH_SLOT_X);
+  launch_copy_slot(st, nv, vi, sdd, vec, LH_SLOT_RHS, LH_SLOT_R);
+  launch_vnorm(st, nv, vi, sdd, gs, 3, vec, LH_SLOT_R, d_norm_.as<double>());
+  launch_zero(st, d_counter_.as<int>());
+  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 1, 0, d_counter_.as<int>());
+  LH_HIP(hipGetLastError());
+  int running = read_counter();
+  while (running > 0) {
+    launch_scale(st, nv, vi, sdd, gs, 1, vec, LH_SLOT_R, 0);
+    for (int j = 0; j < LH_MMAX; ++j) {
+      arnoldi_step(j, mixed);
+      LH_HIP(hipGetLastError());
+      if (read_counter() == 0) break;
+    }
+    launch_backsub(st, S, gs);
+    launch_xupdate(st, nv, vi, sdd, gs, vec);
+    matvec(1, LH_SLOT_X, LH_SLOT_R, 1, false);
+    launch_zero(st, d_counter_.as<int>());
+    launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, d_counter_.as<int>());
+    LH_HIP(hipGetLastError());
+    running = read_counter();
+  }
+  LH_HIP(hipStreamSynchronize(st));
+}
+
+std::vector<SolveResult> Engine::results() {
+  std::vector<GState> h(sd_.size());
+  LH_HIP(hipMemcpyAsync(h.data(), d_state_.p, sizeof(GState) * h.size(), hipMemcpyDeviceToHost,
+                        stream_));
+  LH_HIP(hipStreamSynchronize(stream_));
+  std::vector<SolveResult> r(sd_.size());
+  for (size_t s = 0; s < h.size(); ++s) {
+    const GState& g = h[s];
+    r[s].status = (g.status == 2) ? LH_CONVERGENCE_FAILURE : LH_OK;
+    r[s].iterations = g.total_iters;
+    r[s].cycles = g.cycles;
+    r[s].matvecs = g.matvecs;
+    r[s].matvecs32 = g.matvecs32;
+    r[s].residual = (g.rhs_norm > 0.0) ? g.res_norm / g.rhs_norm : 0.0;
+    r[s].pair_evals = pairs_per_matvec((int)s) * g.matvecs;
+  }
+  return r;
+}
+
+// cda/mod.rs:187-235: C_ext = k/E0^2 sum Im(conj(E) p);
+// C_abs = k/E0^2 sum [Im(p . conj(ainv) . conj(p)) - 2/3 k^3 |p|^2];
+// C_sca = max(C_ext - C_abs, 0).
+std::vector<double> Engine::cross_sections(const std::vector<double>& amplitude) {
+  const int nv = (int)vit_.size();
+  launch_cross(stream_, nv, d_vit_.as<VecItem>(), d_solves_.as<SolveDev>(), d_vec_.as<double2>(),
+               d_ainv_.as<double2>(), d_cross_.as<double>());
+  LH_HIP(hipGetLastError());
+  std::vector<double> part(3 * (size_t)nv);
+  LH_HIP(hipMemcpyAsync(part.data(), d_cross_.p, sizeof(double) * part.size(),
+                        hipMemcpyDeviceToHost, stream_));
+  LH_HIP(hipStreamSynchronize(stream_));
+  std::vector<double> out(3 * sd_.size());
+  for (size_t s = 0; s < sd_.size(); ++s) {
+    const SolveDev& d = sd_[s];
+    double ce = 0.0, ca = 0.0, ps = 0.0;
+    for (int b = 0; b < d.nvi; ++b) {
+      ce += part[3 * (size_t)(d.vi_off + b) + 0];
+      ca += part[3 * (size_t)(d.vi_off + b) + 1];
+      ps += part[3 * (size_t)(d.vi_off + b) + 2];
+    }
+    const double k = d.k;
+    const double e0sq = amplitude[s] * amplitude[s];
+    const double cext = ce * (k / e0sq);
+    const double cabs = (ca - (2.0 / 3.0) * (k * k * k) * ps) * (k / e0sq);
+    out[3 * s + 0] = cext;
+    out[3 * s + 1] = cabs;
+    out[3 * s + 2] = std::max(cext - cabs, 0.0);
+  }
+  return out;
+}
+
+}  // namespace lh
