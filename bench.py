@@ -1,0 +1,248 @@
+"""Benchmark of the CDA solve hot path on one MI355X.
+
+Default workload: 64 Au/Ag spheres and ellipsoids (numpy PCG64 seed 0,
+radius U(5, 30) nm, semi-axis ratios U(1, 2.5), dipole spacing 2 nm, FCD with
+cell_size = spacing) in water (n = 1.33), x-polarised plane wave along +z,
+each at 16 wavelengths 400..800 nm: 1024 GMRES(30) solves of uneven size,
+solved together by the batched device engine in mixed precision (FP32
+far-field pair sum, FP64 Krylov/Givens/residuals/near field).
+
+A step is one batched solve of all 1024 problems (inputs resident in HBM).
+Printed: one JSON line with Green pair-evals/s (the value), solves/s, the
+plain FP64 one-solve-per-launch rate on a subset of the same problems, the
+C_ext error of every mixed result against a batched FP64 solve at tol 1e-12,
+the far-field kernel roofline from HIP events, and the CPU baseline (the
+oracle's FP64 restatement of the reference CPU path on this host's cores).
+
+Other workloads: --workload cuboid (80x40x40 nm Au cuboid, 1.5 nm spacing,
+38,637 dipoles, 520 nm, one solve FP64 and mixed).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X published vector FP32
+FP64_PEAK_TFLOPS = 78.6    # MI355X published vector FP64
+FLOPS_PER_FAR_PAIR = 90    # point-G far pair: y += A x + B d (d . x) (FMA = 2 flops)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def batch_problems(n_particles=64, n_wl=16, n_env=1.33, spacing=2.0):
+    from lumina_amd import inputs as I
+    parts = I.batch_particles(n_particles, seed=0, spacing=spacing)
+    wls = [400.0 + t * 400.0 / (n_wl - 1) for t in range(n_wl)] if n_wl > 1 else [600.0]
+    probs = []
+    for p in parts:
+        for lam in wls:
+            probs.append(dict(positions=p.positions,
+                              polarisability=I.polarisability_block(p.material, lam, spacing,
+                                                                    n_env, depol=p.depol),
+                              k=I.wavenumber(lam, n_env), use_fcd=True, cell_size=spacing,
+                              n=len(p.positions), wl=lam))
+    return probs
+
+
+def pairs(n):
+    return n * (n - 1.0)
+
+
+def cpu_baseline(probs, budget_s=20.0):
+    """The oracle (C++ FP64 restatement of the reference CPU path: rows of
+    the on-the-fly matvec over a thread pool, serial GMRES) on a bounded
+    sample: the batch's problems at the middle wavelength with N <= 2000,
+    smallest first, until the time budget is used."""
+    from oracle import pyoracle as O
+    threads = min(16, os.cpu_count() or 1)
+    mid = sorted(set(p["wl"] for p in probs))[len(set(p["wl"] for p in probs)) // 2]
+    sample = sorted([p for p in probs if p["wl"] == mid and p["n"] <= 2000], key=lambda p: p["n"])
+    t0 = time.perf_counter()
+    ev, nsolve = 0.0, 0
+    for p in sample:
+        P = O.Problem(p["positions"], p["polarisability"], p["k"], True, p["cell_size"])
+        _, _, info = P.compute_cross_sections(1e-8, 1000, nthreads=threads)
+        ev += info["matvecs"] * pairs(p["n"])
+        nsolve += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": ev / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+            "solves_per_s": nsolve / dt,
+            "sample": f"{nsolve} full GMRES solves (tol 1e-8, FP64, FCD) of the batch's "
+                      f"problems at {mid:.1f} nm with N <= 2000, smallest first; "
+                      f"{ev:.3g} pair-evals in {dt:.1f} s"}
+
+
+def run_batch(args):
+    from lumina_amd.solver import FP64, MIXED, Batch, HipBackend, IncidentField, PairSumSession
+    probs = batch_problems(args.particles, args.wavelengths)
+    S = len(probs)
+    log(f"[bench] batch: {S} solves, N in [{min(p['n'] for p in probs)}, "
+        f"{max(p['n'] for p in probs)}], total dipoles {sum(p['n'] for p in probs)}")
+    be = HipBackend(0)
+    t0 = time.perf_counter()
+    batch = Batch(be, probs)
+    log(f"[bench] batch resident in HBM ({time.perf_counter() - t0:.1f} s to build)")
+    batch.set_timing(True)
+
+    prec = MIXED if args.precision == "mixed" else FP64
+    for w in range(args.warmup):
+        batch.run(args.tol, args.max_iter, prec)
+        log(f"[bench] warmup {w} done")
+    times, evals, solves = [], 0.0, 0
+    kt_tot = {"far32_ms": 0.0, "far32_pairs": 0.0, "far32_launches": 0,
+              "far64_ms": 0.0, "far64_pairs": 0.0, "far64_launches": 0}
+    for s in range(args.steps):
+        t1 = time.perf_counter()
+        batch.run(args.tol, args.max_iter, prec)
+        dt = time.perf_counter() - t1
+        times.append(dt)
+        cs_mixed, infos, _ = batch.results()
+        evals += sum(i["pair_evals"] for i in infos)
+        solves += S
+        kt = batch.kernel_times()
+        for key in kt_tot:
+            kt_tot[key] += kt[key]
+        log(f"[bench] step {s}: {dt:.2f} s, iters mean "
+            f"{np.mean([i['iterations'] for i in infos]):.1f}, max {max(i['iterations'] for i in infos)}")
+    total = sum(times)
+    value = evals / total
+    iters = [i["iterations"] for i in infos]
+
+    # reference C_ext: batched FP64 at tol 1e-12
+    err = None
+    if not args.no_error:
+        batch.run(1e-12, args.max_iter, FP64)
+        cs_ref, _, _ = batch.results()
+        err = float(np.max(np.abs(cs_mixed[:, 0] - cs_ref[:, 0]) / np.abs(cs_ref[:, 0])))
+        log(f"[bench] max C_ext err vs FP64 (tol 1e-12): {err:.3e}")
+    batch.close()
+
+    # plain FP64: one solve per launch sequence, a subset of the same problems
+    plain = None
+    if not args.no_fp64_plain:
+        sub = probs[args.wavelengths // 2::args.wavelengths][: args.plain_count]
+        t_pl, ev_pl = 0.0, 0.0
+        for p in sub:
+            s_ = PairSumSession(be, p["positions"], p["polarisability"], p["k"], True,
+                                p["cell_size"], precision=FP64)
+            t1 = time.perf_counter()
+            out, _, info = s_.compute_cross_sections(IncidentField(), args.tol, args.max_iter)
+            t_pl += time.perf_counter() - t1
+            ev_pl += info["pair_evals"]
+            s_.close()
+        plain = {"pair_evals_per_s": ev_pl / t_pl, "solves_per_s": len(sub) / t_pl,
+                 "solves": len(sub),
+                 "sample": f"{len(sub)} problems (each particle at the middle wavelength), "
+                           f"one FP64 session and solve each, tol {args.tol:g}"}
+        log(f"[bench] plain FP64: {plain['pair_evals_per_s']:.3e} pair-evals/s")
+
+    far_ms = kt_tot["far32_ms"] if prec == MIXED else kt_tot["far64_ms"]
+    far_pairs = kt_tot["far32_pairs"] if prec == MIXED else kt_tot["far64_pairs"]
+    far_n = kt_tot["far32_launches"] if prec == MIXED else kt_tot["far64_launches"]
+    achieved = FLOPS_PER_FAR_PAIR * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else 0.0
+    peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
+    roof = {"bound": "valu", "kernel": "k_far32<false>" if prec == MIXED else "k_far64<false>",
+            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
+            "flops_per_pair": FLOPS_PER_FAR_PAIR,
+            "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0}
+
+    line = {
+        "metric": "Green pair-evals/s (batched mixed-precision CDA solves)",
+        "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 far field + fp64 Krylov" if prec == MIXED else "fp64",
+        "data": "synthetic (seeded discretised particles, J&C Au/Ag)",
+        "config": {"workload": f"{args.particles} Au/Ag spheres+ellipsoids x {args.wavelengths} "
+                               f"wavelengths 400-800 nm, d=2 nm, FCD, water, GMRES(30) "
+                               f"tol {args.tol:g}, one batched solve per step",
+                   "solves": S, "dipoles": int(sum(p["n"] for p in probs)),
+                   "n_min": int(min(p["n"] for p in probs)),
+                   "n_max": int(max(p["n"] for p in probs)), "parallelism": "single GPU"},
+        "solves_per_s": solves / total,
+        "iterations_mean": float(np.mean(iters)), "iterations_max": int(max(iters)),
+        "cext_rel_err_max_vs_fp64": err,
+        "fp64_plain": plain,
+        "speedup_vs_fp64_plain_pair_evals": (value / plain["pair_evals_per_s"]) if plain else None,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(probs)
+        log(f"[bench] cpu baseline {line['cpu_baseline']['value']:.3e} pair-evals/s")
+    return line
+
+
+def run_cuboid(args):
+    from lumina_amd import inputs as I
+    from lumina_amd.solver import FP64, MIXED, IncidentField, PairSumSession, HipBackend
+    pos = I.au_cuboid_80x40x40_d1p5()
+    lam, n_env, d = 520.0, 1.33, 1.5
+    alpha = I.polarisability_block("Au", lam, d, n_env)
+    k = I.wavenumber(lam, n_env)
+    be = HipBackend(0)
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        s = PairSumSession(be, pos, alpha, k, True, d, precision=prec)
+        for _ in range(args.warmup):
+            s.compute_cross_sections(IncidentField(), args.tol, args.max_iter)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out, p, info = s.compute_cross_sections(IncidentField(), args.tol, args.max_iter)
+        dt = (time.perf_counter() - t0) / args.steps
+        res[name] = {"cs": out.tolist(), "s": dt, "pair_evals_per_s": info["pair_evals"] / dt,
+                     "info": info}
+        log(f"[bench] cuboid {name}: {dt:.2f} s, {info}")
+        s.close()
+    err = abs(res["mixed"]["cs"][0] - res["fp64"]["cs"][0]) / abs(res["fp64"]["cs"][0])
+    return {
+        "metric": "Green pair-evals/s (single large mixed-precision CDA solve)",
+        "value": res["mixed"]["pair_evals_per_s"], "unit": "pair-evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * res["mixed"]["s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 far field + fp64 Krylov", "data": "synthetic",
+        "config": {"workload": "Au cuboid 80x40x40 nm, d=1.5 nm (38,637 dipoles), 520 nm, water, "
+                               f"FCD, GMRES(30) tol {args.tol:g}", "parallelism": "single GPU"},
+        "fp64": res["fp64"], "mixed": res["mixed"], "cext_rel_err_mixed_vs_fp64": err,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["batch", "cuboid"], default="batch")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iter", type=int, default=2000)
+    ap.add_argument("--particles", type=int, default=64)
+    ap.add_argument("--wavelengths", type=int, default=16)
+    ap.add_argument("--plain-count", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-plain", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding "
+              f"of this path is a design (see DESIGN.md) and is not built.", file=sys.stderr)
+        sys.exit(2)
+    line = run_batch(args) if args.workload == "batch" else run_cuboid(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -188,6 +188,31 @@ int lh_batch_solve(lh_backend* backend, const lh_batch_problem* problems, size_t
                    double tol, size_t max_iter, int precision, double* cross_sections,
                    lh_c64* const* moments, lh_solve_info* infos);
 
+/* The same, split so that a batch stays resident on the device and can be
+ * solved repeatedly: create uploads geometry, alpha^-1, near tensors and
+ * incident fields; run solves every problem; results reads them back. */
+typedef struct lh_batch lh_batch;
+int lh_batch_create(lh_backend* backend, const lh_batch_problem* problems, size_t count,
+                    lh_batch** out);
+int lh_batch_run(lh_batch* batch, double tol, size_t max_iter, int precision);
+int lh_batch_results(lh_batch* batch, double* cross_sections, lh_c64* const* moments,
+                     lh_solve_info* infos);
+void lh_batch_destroy(lh_batch* batch);
+
+/* Instrumentation (no reference counterpart): when enabled, the far-field
+ * pair-sum launches of lh_batch_run are bracketed by HIP events on the
+ * batch's stream and their device time is accumulated. pairs_* count the
+ * Green's pair evaluations those launches performed (N(N-1), plus N^2 image
+ * pairs, per participating solve). */
+typedef struct {
+  double far64_ms, far32_ms;
+  uint64_t far64_launches, far32_launches;
+  double far64_pairs, far32_pairs;
+  double run_ms;          /* wall time of the last lh_batch_run */
+} lh_kernel_times;
+int lh_batch_set_timing(lh_batch* batch, int enable);
+int lh_batch_kernel_times(lh_batch* batch, lh_kernel_times* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,7 @@
 // C ABI (include/lumina_hip.h): backend, stored-matrix session, on-the-fly
 // pair-sum sessions and batched solves.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -152,6 +153,14 @@ struct lh_pairsum {
   int precision = 0;
   lh_c64 delta{0.0, 0.0};
   int has_sub = 0;
+};
+
+struct lh_batch {
+  lh_backend* b = nullptr;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<lh::Engine> eng;
+  std::vector<double> amp;
+  bool ran = false;
 };
 
 namespace lh {
@@ -394,14 +403,11 @@ void lh_pairsum_destroy(lh_pairsum* s) {
 
 // ---- batch ----------------------------------------------------------------
 
-int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count, double tol,
-                   size_t max_iter, int precision, double* cross_sections,
-                   lh_c64* const* moments, lh_solve_info* infos) {
+int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t count,
+                    lh_batch** out) {
   return guarded([&]() {
-    if (!b || !problems || count == 0 || !cross_sections)
-      return fail(LH_INVALID_ARGUMENT, "bad argument");
-    if (precision != LH_PREC_FP64 && precision != LH_PREC_MIXED)
-      return fail(LH_INVALID_ARGUMENT, "unknown precision");
+    if (!b || !problems || count == 0 || !out) return fail(LH_INVALID_ARGUMENT, "bad argument");
+    *out = nullptr;
     LH_HIP(hipSetDevice(b->device));
     struct Key {
       const double* p;
@@ -412,11 +418,14 @@ int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count
     std::vector<Key> keys;
     std::vector<lh::Geometry> geoms;
     std::vector<lh::Problem> probs;
-    std::vector<double> amp;
+    auto bt = std::make_unique<lh_batch>();
+    bt->b = b;
     for (size_t q = 0; q < count; ++q) {
       lh_pairsum_spec sp = problems[q].spec;
-      sp.precision = precision;
+      sp.precision = LH_PREC_FP64;
       check_spec(&sp);
+      if (!(problems[q].incident.amplitude != 0.0))
+        return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
       int gi = -1;
       for (size_t t = 0; t < keys.size(); ++t)
         if (keys[t].p == sp.positions && keys[t].n == sp.n_dipoles &&
@@ -430,33 +439,104 @@ int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count
         geoms.push_back(make_geometry(&sp));
       }
       probs.push_back(make_problem(&sp, gi));
-      amp.push_back(problems[q].incident.amplitude);
-      if (!(problems[q].incident.amplitude != 0.0))
-        return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+      bt->amp.push_back(problems[q].incident.amplitude);
     }
-    hipStream_t st;
-    LH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
-      hipStream_t s;
-      ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    lh::Engine eng(st);
+    LH_HIP(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+    bt->eng = std::make_unique<lh::Engine>(bt->stream);
     lh::Tiling t;
-    eng.build(geoms, probs, t);
-    for (size_t q = 0; q < count; ++q) eng.fill_incident((int)q, problems[q].incident);
-    eng.solve(tol, (long long)max_iter, precision == LH_PREC_MIXED, true);
-    const auto res = eng.results();
-    const auto cs = eng.cross_sections(amp);
+    bt->eng->build(geoms, probs, t);
+    for (size_t q = 0; q < count; ++q) bt->eng->fill_incident((int)q, problems[q].incident);
+    *out = bt.release();
+    return (int)LH_OK;
+  });
+}
+
+int lh_batch_run(lh_batch* bt, double tol, size_t max_iter, int precision) {
+  return guarded([&]() {
+    if (!bt) return fail(LH_INVALID_ARGUMENT, "null batch");
+    if (precision != LH_PREC_FP64 && precision != LH_PREC_MIXED)
+      return fail(LH_INVALID_ARGUMENT, "unknown precision");
+    if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
+    LH_HIP(hipSetDevice(bt->b->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    bt->eng->solve(tol, (long long)max_iter, precision == LH_PREC_MIXED, true);
+    const auto t1 = std::chrono::steady_clock::now();
+    bt->eng->last_run_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    bt->ran = true;
+    return (int)LH_OK;
+  });
+}
+
+int lh_batch_results(lh_batch* bt, double* cross_sections, lh_c64* const* moments,
+                     lh_solve_info* infos) {
+  return guarded([&]() {
+    if (!bt || !cross_sections) return fail(LH_INVALID_ARGUMENT, "bad argument");
+    if (!bt->ran) return fail(LH_INVALID_ARGUMENT, "lh_batch_run has not been called");
+    LH_HIP(hipSetDevice(bt->b->device));
+    const auto res = bt->eng->results();
+    const auto cs = bt->eng->cross_sections(bt->amp);
     std::memcpy(cross_sections, cs.data(), sizeof(double) * cs.size());
     bool all_ok = true;
-    for (size_t q = 0; q < count; ++q) {
+    for (size_t q = 0; q < res.size(); ++q) {
       if (infos) fill_info(res[q], &infos[q]);
-      if (moments && moments[q]) eng.download_slot((int)q, LH_SLOT_X, moments[q]);
+      if (moments && moments[q]) bt->eng->download_slot((int)q, LH_SLOT_X, moments[q]);
       if (res[q].status != LH_OK) all_ok = false;
     }
     if (!all_ok) return fail(LH_CONVERGENCE_FAILURE, "at least one solve did not converge");
     return (int)LH_OK;
   });
+}
+
+void lh_batch_destroy(lh_batch* bt) {
+  if (!bt) return;
+  (void)hipSetDevice(bt->b->device);
+  bt->eng.reset();
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  delete bt;
+}
+
+int lh_batch_set_timing(lh_batch* bt, int enable) {
+  return guarded([&]() {
+    if (!bt) return fail(LH_INVALID_ARGUMENT, "null batch");
+    bt->eng->set_timing(enable != 0);
+    return (int)LH_OK;
+  });
+}
+
+int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
+  return guarded([&]() {
+    if (!bt || !out) return fail(LH_INVALID_ARGUMENT, "bad argument");
+    LH_HIP(hipSetDevice(bt->b->device));
+    double t[4];
+    bt->eng->far_times(t);
+    std::memset(out, 0, sizeof(*out));
+    out->far64_ms = t[0];
+    out->far32_ms = t[1];
+    out->far64_launches = (uint64_t)t[2];
+    out->far32_launches = (uint64_t)t[3];
+    const auto res = bt->eng->results();
+    for (size_t q = 0; q < res.size(); ++q) {
+      const double p = bt->eng->pairs_per_matvec((int)q);
+      out->far32_pairs += p * res[q].matvecs32;
+      out->far64_pairs += p * (res[q].matvecs - res[q].matvecs32);
+    }
+    out->run_ms = bt->eng->last_run_ms;
+    return (int)LH_OK;
+  });
+}
+
+int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count, double tol,
+                   size_t max_iter, int precision, double* cross_sections,
+                   lh_c64* const* moments, lh_solve_info* infos) {
+  lh_batch* bt = nullptr;
+  int st = lh_batch_create(b, problems, count, &bt);
+  if (st != LH_OK) return st;
+  st = lh_batch_run(bt, tol, max_iter, precision);
+  if (st == LH_OK) st = lh_batch_results(bt, cross_sections, moments, infos);
+  std::string msg = g_last_error;
+  lh_batch_destroy(bt);
+  g_last_error = msg;
+  return st;
 }
 
 }  // extern "C"

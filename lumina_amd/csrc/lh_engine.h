@@ -96,7 +96,21 @@ class Engine {
   double pairs_per_matvec(int s) const;
   hipStream_t stream() const { return stream_; }
 
+  // Far-field launch timing (HIP events on the engine's stream).
+  void set_timing(bool on) { timing_ = on; }
+  // Sums over the far launches since the last solve() started:
+  // {far64_ms, far32_ms, far64_launches, far32_launches}.
+  void far_times(double out[4]);
+  double last_run_ms = 0.0;
+
  private:
+  struct EvPair {
+    hipEvent_t a, b;
+    bool fp32;
+  };
+  std::vector<EvPair> evs_;
+  bool timing_ = false;
+  void clear_events();
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
   void arnoldi_step(int j, bool fp32);
   int read_counter();

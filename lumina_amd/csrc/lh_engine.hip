@@ -141,6 +141,7 @@ void build_near_list(Geometry& g) {
 // ---------------------------------------------------------------------------
 
 Engine::~Engine() {
+  clear_events();
   for (int j = 0; j < LH_MMAX; ++j)
     for (int f = 0; f < 2; ++f)
       if (graph_ready_[j][f]) (void)hipGraphExecDestroy(step_graph_[j][f]);
@@ -380,9 +381,15 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
+  EvPair ev{nullptr, nullptr, fp32};
+  if (timing_) {
+    LH_HIP(hipEventCreate(&ev.a));
+    LH_HIP(hipEventCreate(&ev.b));
+  }
   if (fp32) {
     launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
+    if (timing_) LH_HIP(hipEventRecord(ev.a, st));
     if (any_free_)
       launch_far32(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
                    d_py_.as<double>(), d_pz_.as<double>(), d_src32_.as<Src32>(),
@@ -394,6 +401,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
+    if (timing_) LH_HIP(hipEventRecord(ev.a, st));
     if (any_free_)
       launch_far64(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
                    d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
@@ -402,6 +410,10 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
       launch_far64(st, true, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
                    d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
                    d_part_.as<double2>());
+  }
+  if (timing_) {
+    LH_HIP(hipEventRecord(ev.b, st));
+    evs_.push_back(ev);
   }
   launch_reduce(st, nv, vi, sd, gs, which, d_rp_.as<long long>(), d_col_.as<int>(),
                 d_t_.as<double>(), t_stride_, d_ainv_.as<double2>(), vec, in_slot, out_slot,
@@ -439,8 +451,28 @@ void Engine::arnoldi_step(int j, bool fp32) {
   launch_scale(st, nv, vi, sd, gs, 2, vec, LH_SLOT_W, -1);
 }
 
+void Engine::clear_events() {
+  for (auto& e : evs_) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  evs_.clear();
+}
+
+void Engine::far_times(double out[4]) {
+  LH_HIP(hipStreamSynchronize(stream_));
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  for (auto& e : evs_) {
+    float ms = 0.0f;
+    LH_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+    out[e.fp32 ? 1 : 0] += ms;
+    out[e.fp32 ? 3 : 2] += 1.0;
+  }
+}
+
 void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   (void)use_graph;
+  clear_events();
   hipStream_t st = stream_;
   const int nv = (int)vit_.size(), S = (int)sd_.size();
   const VecItem* vi = d_vit_.as<VecItem>();
@@ -461,7 +493,10 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   launch_check(st, S, sdd, gs, d_norm_.as<double>(), 1, 0, d_counter_.as<int>());
   LH_HIP(hipGetLastError());
   int running = read_counter();
-  while (running > 0) {
+  // every cycle performs at least one Arnoldi step of every running solve,
+  // so at most max_iter + 1 cycles can start
+  long long cycles = 0;
+  while (running > 0 && cycles++ <= max_iter + 1) {
     launch_scale(st, nv, vi, sdd, gs, 1, vec, LH_SLOT_R, 0);
     for (int j = 0; j < LH_MMAX; ++j) {
       arnoldi_step(j, mixed);

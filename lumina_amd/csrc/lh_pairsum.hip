@@ -143,8 +143,8 @@ __global__ __launch_bounds__(LH_VB) void k_prep(const VecItem* __restrict__ item
   r.z = pz[S.pos_off + j];
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    r.v[q] = (decltype(r.v[0]))v[q];
-    r.w[q] = (decltype(r.w[0]))w[q];
+    r.v[q] = static_cast<typename SRC::value_type>(v[q]);
+    r.w[q] = static_cast<typename SRC::value_type>(w[q]);
   }
   src[S.src_off + j] = r;
 }

@@ -104,12 +104,14 @@ struct GState {
 // vector of one dipole, read by every lane of a wave through the scalar
 // cache.
 struct Src64 {
+  typedef double value_type;
   double x, y, z, pad;
   double v[6];          // x_j (re, im) x 3
   double w[6];          // (-D, -D, +D) o x_j for the image term
 };
 
 struct Src32 {
+  typedef float value_type;
   double x, y, z;       // FP64: exclusion test and displacement
   float v[6];
   float w[6];

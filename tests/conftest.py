@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def backend():
+    """The HIP backend on device 0. No fallback: fails if the library or the
+    GPU is missing."""
+    from lumina_amd.solver import HipBackend
+    b = HipBackend(0)
+    yield b
+    b.close()

@@ -1,0 +1,213 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Bars: FP64 matvec <= 1e-12 relative (max-norm) against the oracle's
+on-the-fly matvec; FP64 solve C_ext/C_abs/C_sca within 1e-10 relative of the
+oracle's GMRES at tol 1e-12; FP32 far-field matvec within 2e-6 relative;
+mixed-precision solve C_ext within 1e-6 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from lumina_amd import inputs as I
+from lumina_amd.solver import (FP64, MIXED, ConvergenceFailure, IncidentField, SubstrateRuntime,
+                               batch_compute)
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def sphere_problem(lam=520.0, n_env=1.33, mat="Au", cell=2.0):
+    pos = I.au_sphere_r10_d2()
+    alpha = I.polarisability_block(mat, lam, 2.0, n_env)
+    k = I.wavenumber(lam, n_env)
+    return pos, alpha, k, cell
+
+
+def rand_x(n, seed=0):
+    r = np.random.default_rng(seed)
+    return r.standard_normal(3 * n) + 1j * r.standard_normal(3 * n)
+
+
+def test_device_info(backend):
+    info = backend.device_info()
+    print(info)
+    assert info["backend_type"] == "Gpu"
+    assert "gfx950" in info["arch"]
+
+
+@pytest.mark.parametrize("use_fcd", [True, False])
+def test_matvec_fp64_sphere(backend, use_fcd):
+    pos, alpha, k, cell = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=use_fcd, cell_size=cell)
+    x = rand_x(len(pos))
+    y = s.matvec(x, FP64)
+    yo = O.Problem(pos, alpha, k, use_fcd=use_fcd, cell_size=cell).matvec(x)
+    assert rel(y, yo) < 1e-12
+
+
+def test_matvec_fp64_cli_cell_size(backend):
+    # CLI default cell_size = 3.0 with spacing 2 (cda/mod.rs:89): 122 near pairs per dipole
+    pos, alpha, k, _ = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=3.0)
+    x = rand_x(len(pos), 1)
+    assert rel(s.matvec(x, FP64), O.Problem(pos, alpha, k, True, 3.0).matvec(x)) < 1e-12
+
+
+def test_matvec_substrate_and_anisotropic(backend):
+    pos = I.discretise(I.Ellipsoid((12.0, 8.0, 6.0)), 2.0)
+    pos = pos.copy()
+    pos[:, 2] += 10.0
+    lam, n_env = 600.0, 1.0
+    k = I.wavenumber(lam, n_env)
+    depol = I.depol_factors((12.0, 8.0, 6.0))
+    a = I.polarisability_block("Ag", lam, 2.0, n_env, depol=depol)
+    # per-dipole anisotropic blocks with a small variation
+    blocks = np.tile(a, (len(pos), 1)) * (1.0 + 0.01 * np.arange(len(pos)))[:, None]
+    sub = SubstrateRuntime(z_interface=-1.0, delta_eps=I_delta())
+    s = backend.create_pairsum_session(pos, blocks, k, use_fcd=True, cell_size=2.0, substrate=sub)
+    x = rand_x(len(pos), 2)
+    yo = O.Problem(pos, blocks, k, True, 2.0, substrate=(-1.0, sub.delta_eps)).matvec(x)
+    assert rel(s.matvec(x, FP64), yo) < 1e-12
+
+
+def I_delta():
+    from lumina_amd.solver import substrate_reflection_factor
+    return substrate_reflection_factor(2.25 + 0.0j, 1.0, True)
+
+
+def test_matvec_random_positions(backend):
+    # off-grid positions, no coincident pairs, FCD radius crossing many pairs
+    r = np.random.default_rng(3)
+    pos = r.uniform(-8, 8, size=(300, 3))
+    alpha = I.isotropic_block(50.0 + 20.0j)
+    k = 2 * np.pi / 500.0
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.7)
+    x = rand_x(300, 4)
+    assert rel(s.matvec(x, FP64), O.Problem(pos, alpha, k, True, 1.7).matvec(x)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 257])
+def test_matvec_small_and_ragged(backend, n):
+    r = np.random.default_rng(n)
+    pos = r.uniform(-20, 20, size=(n, 3))
+    alpha = I.isotropic_block(30.0 + 5.0j)
+    k = 2 * np.pi / 450.0
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=2.0)
+    x = rand_x(n, 5)
+    assert rel(s.matvec(x, FP64), O.Problem(pos, alpha, k, True, 2.0).matvec(x)) < 1e-12
+
+
+def test_matvec_fp32_far_field(backend):
+    pos, alpha, k, cell = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    x = rand_x(len(pos), 6)
+    yo = O.Problem(pos, alpha, k, True, cell).matvec(x)
+    e = rel(s.matvec(x, MIXED), yo)
+    print("fp32 far-field matvec rel err", e)
+    assert e < 2e-6
+
+
+def test_solve_fp64_parity_sphere(backend):
+    pos, alpha, k, cell = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    out, p, info = s.compute_cross_sections(IncidentField(), 1e-12, 1000)
+    P = O.Problem(pos, alpha, k, True, cell)
+    cso, po, infoo = P.compute_cross_sections(1e-12, 1000)
+    print("gpu", out, info, "oracle", cso, infoo)
+    assert np.max(np.abs(out - cso) / np.abs(cso)) < 1e-10
+    assert np.linalg.norm(p - po) / np.linalg.norm(po) < 1e-9
+
+
+def test_solve_fp64_tol_1e10(backend):
+    pos, alpha, k, cell = sphere_problem(lam=450.0)
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    out, p, info = s.compute_cross_sections(IncidentField(), 1e-10, 1000)
+    cso, po, infoo = O.Problem(pos, alpha, k, True, cell).compute_cross_sections(1e-10, 1000)
+    print("tol 1e-10", out, cso, info, infoo)
+    assert np.max(np.abs(out - cso) / np.abs(cso)) < 1e-8
+    assert info["residual"] < 1e-10
+
+
+def test_solve_mixed_error(backend):
+    pos, alpha, k, cell = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell,
+                                       precision=MIXED)
+    out, p, info = s.compute_cross_sections(IncidentField(), 1e-10, 1000)
+    cso, _, _ = O.Problem(pos, alpha, k, True, cell).compute_cross_sections(1e-12, 1000)
+    err = abs(out[0] - cso[0]) / abs(cso[0])
+    print("mixed C_ext err", err, info)
+    assert err < 1e-6
+    assert info["matvecs_fp32"] > 0
+
+
+def test_convergence_failure(backend):
+    pos, alpha, k, cell = sphere_problem()
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    with pytest.raises(ConvergenceFailure) as ei:
+        s.compute_cross_sections(IncidentField(), 1e-12, 5)
+    assert ei.value.info["iterations"] == 5
+
+
+def test_batch_matches_single(backend):
+    pos = I.au_sphere_r10_d2()
+    pos2 = I.discretise(I.Ellipsoid((8.0, 5.0, 4.0)), 2.0)
+    probs = []
+    for lam in (450.0, 520.0, 610.0):
+        k = I.wavenumber(lam, 1.33)
+        probs.append(dict(positions=pos, polarisability=I.polarisability_block("Au", lam, 2.0, 1.33),
+                          k=k, use_fcd=True, cell_size=2.0))
+        d = I.depol_factors((8.0, 5.0, 4.0))
+        probs.append(dict(positions=pos2,
+                          polarisability=I.polarisability_block("Ag", lam, 2.0, 1.33, depol=d),
+                          k=k, use_fcd=True, cell_size=2.0))
+    cs, infos, moms = batch_compute(backend, probs, 1e-12, 1000, precision=FP64, want_moments=True)
+    for q, pr in enumerate(probs):
+        P = O.Problem(pr["positions"], pr["polarisability"], pr["k"], True, 2.0)
+        cso, po, _ = P.compute_cross_sections(1e-12, 1000)
+        assert np.max(np.abs(cs[q] - cso) / np.abs(cso)) < 1e-10, (q, cs[q], cso)
+
+
+def test_batch_mixed(backend):
+    pos = I.au_sphere_r10_d2()
+    probs = []
+    for lam in (400.0, 500.0, 700.0):
+        k = I.wavenumber(lam, 1.33)
+        probs.append(dict(positions=pos, polarisability=I.polarisability_block("Au", lam, 2.0, 1.33),
+                          k=k, use_fcd=True, cell_size=2.0))
+    cs, infos, _ = batch_compute(backend, probs, 1e-10, 1000, precision=MIXED)
+    for q, pr in enumerate(probs):
+        cso, _, _ = O.Problem(pos, pr["polarisability"], pr["k"], True, 2.0).compute_cross_sections(1e-12, 1000)
+        assert abs(cs[q][0] - cso[0]) / abs(cso[0]) < 1e-6
+
+
+def test_stored_matrix_session(backend):
+    r = np.random.default_rng(7)
+    n = 300
+    A = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    x = r.standard_normal(n) + 1j * r.standard_normal(n)
+    s = backend.create_session(n)
+    s.upload_matrix(A)
+    assert rel(s.matvec(x), A @ x) < 1e-13
+
+
+def test_large_cuboid_rows(backend):
+    """38,637-dipole cuboid: FP64 and FP32 matvec rows against the oracle on a
+    sample of rows (the oracle's cost is N per row)."""
+    pos = I.au_cuboid_80x40x40_d1p5()
+    lam, n_env = 520.0, 1.33
+    alpha = I.polarisability_block("Au", lam, 1.5, n_env)
+    k = I.wavenumber(lam, n_env)
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.5)
+    n = len(pos)
+    x = rand_x(n, 8)
+    y64 = s.matvec(x, FP64)
+    y32 = s.matvec(x, MIXED)
+    P = O.Problem(pos, alpha, k, True, 1.5)
+    for lo in (0, n // 2, n - 64):
+        yo = P.matvec_rows(x, lo, lo + 64)
+        assert rel(y64[3 * lo:3 * lo + 192], yo) < 1e-12
+        assert rel(y32[3 * lo:3 * lo + 192], yo) < 5e-6

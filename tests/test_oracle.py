@@ -1,0 +1,326 @@
+"""Pins the oracle (CPU FP64 restatement) before it is trusted as the
+parity reference: golden vectors derived independently (mpmath), the
+reference's own known-answer tests, closed forms, LAPACK, and invariants.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.special
+
+from oracle import pyoracle as O
+from lumina_amd import inputs as I
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _t(c):
+    a = np.array(c)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+# ---- point Green's tensor (greens.rs:34-71) --------------------------------
+
+def test_point_tensor_vs_mpmath_derivatives():
+    cases = json.load(open(os.path.join(GOLD, "greens_point.json")))
+    for c in cases:
+        G = _t(c["G"])
+        g = O.greens_point(c["r1"], c["r2"], c["k"])
+        assert np.max(np.abs(g - G)) / np.max(np.abs(G)) < 1e-13, c
+
+
+def test_point_tensor_reciprocity():
+    # greens.rs:175-190
+    k = 2 * np.pi / 500.0
+    g12 = O.greens_point([0, 0, 0], [10, 5, 3], k)
+    g21 = O.greens_point([10, 5, 3], [0, 0, 0], k)
+    assert np.max(np.abs(g12 - g21.T)) < 1e-12
+
+
+def test_point_tensor_far_field_limit():
+    # kR >> 1: G -> k^2 e^{ikR}/(4 pi R) (I - R^ R^); transverse, trace 2x
+    k = 2 * np.pi / 500.0
+    R = 5e6
+    g = O.greens_point([R, 0, 0], [0, 0, 0], k)
+    pf = k * k * np.exp(1j * k * R) / (4 * np.pi * R)
+    # longitudinal part ~ 2/(kR) pf, transverse correction ~ 1/(kR) pf
+    assert abs(g[0, 0]) < 4.0 / (k * R) * abs(pf)
+    assert abs(g[1, 1] - pf) < 2.0 / (k * R) * abs(pf)
+
+
+# ---- FCD (greens.rs:94-167) ------------------------------------------------
+
+def test_fcd_vs_mpmath_quadrature():
+    cases = json.load(open(os.path.join(GOLD, "greens_fcd.json")))
+    for c in cases:
+        G = _t(c["G"])
+        g = O.greens_filtered(c["r_obs"], c["r_src"], c["k"], c["cell"])
+        assert np.max(np.abs(g - G)) / np.max(np.abs(G)) < 1e-12, c
+
+
+def test_fcd_far_equals_point():
+    # greens.rs:201-217: at 10 cells FCD == point G (beyond 2 cells it is point G)
+    k = 2 * np.pi / 500.0
+    assert np.max(np.abs(O.greens_filtered([0, 0, 0], [30, 0, 0], k, 3.0)
+                         - O.greens_point([0, 0, 0], [30, 0, 0], k))) < 1e-10
+
+
+def test_fcd_near_finite_and_different():
+    # greens.rs:220-261
+    k = 2 * np.pi / 500.0
+    f = O.greens_filtered([0, 0, 0], [3, 0, 0], k, 3.0)
+    p = O.greens_point([0, 0, 0], [3, 0, 0], k)
+    assert np.all(np.isfinite(f))
+    assert np.max(np.abs(f - p)) > 1e-8
+
+
+def test_fcd_boundary_is_inclusive():
+    # dist == 2 cell is near (the test is dist > 2 cell for point G)
+    k = 2 * np.pi / 500.0
+    f = O.greens_filtered([0, 0, 0], [4, 0, 0], k, 2.0)
+    p = O.greens_point([0, 0, 0], [4, 0, 0], k)
+    assert np.max(np.abs(f - p)) > 1e-9
+
+
+# ---- Ewald 2D (ewald.rs:91-338), lattice (lattice.rs) -----------------------
+
+def test_erfc_as_7_1_26():
+    # ewald.rs:816-824 and the A&S bound |err| < 1.5e-7 against scipy
+    assert abs(O.erfc_real(0.0) - 1.0) < 1e-7
+    assert O.erfc_real(10.0) < 1e-40
+    assert abs(O.erfc_real(1.5) + O.erfc_real(-1.5) - 2.0) < 1e-6
+    xs = np.linspace(0, 6, 601)
+    err = max(abs(O.erfc_real(x) - scipy.special.erfc(x)) for x in xs)
+    assert err < 1.5e-7
+    assert err > 1e-9  # it is the approximation, not libm erfc
+
+
+def test_csqrt_branch():
+    # ewald.rs:827-838
+    s = O.csqrt_positive_imag(-4.0)
+    assert abs(s - 2j) < 1e-12
+    assert abs(O.csqrt_positive_imag(9.0) - 3.0) < 1e-12
+    s = O.csqrt_positive_imag(complex(-3.0, -1e-3))
+    assert s.imag >= 0 and abs(s * s - complex(-3.0, -1e-3)) < 1e-12
+
+
+def test_lattice_helpers():
+    # lattice.rs:243-301
+    a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
+    b1, b2 = O.recip_vectors(a1, a2)
+    assert np.allclose(b1, [2 * np.pi / 100, 0, 0], atol=1e-14)
+    assert np.allclose(b2, [0, 2 * np.pi / 100, 0], atol=1e-14)
+    assert abs(O.lattice_eta(a1, a2) - np.sqrt(np.pi / 1e4)) < 1e-15
+    assert O.lattice_eta(a1, a2, 0.05) == 0.05
+
+
+def test_ewald_split_identity():
+    # ewald.rs:858-872
+    a1, a2 = [80.0, 0, 0], [0, 80.0, 0]
+    k = 2 * np.pi / 500.0
+    r = [10.0, 5.0, 2.0]
+    full = O.ewald(a1, a2, r, [0, 0, 0], k, 0)
+    assert np.max(np.abs(full - O.ewald(a1, a2, r, [0, 0, 0], k, 1)
+                         - O.ewald(a1, a2, r, [0, 0, 0], k, 2))) < 1e-20
+
+
+def test_ewald_gamma_reciprocity():
+    # ewald.rs:879-897
+    a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
+    k = 2 * np.pi / 600.0
+    g = O.ewald(a1, a2, [15.0, 8.0, 3.0], [0, 0, 0], k)
+    gn = O.ewald(a1, a2, [-15.0, -8.0, -3.0], [0, 0, 0], k)
+    assert np.max(np.abs(g - gn.T)) < 1e-8
+
+
+def test_ewald_square_self_image_symmetry():
+    # ewald.rs:902-919
+    a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
+    k = 2 * np.pi / 600.0
+    g = O.ewald(a1, a2, [0, 0, 0], [0, 0, 0], k)
+    assert np.all(np.isfinite(g))
+    assert abs(g[0, 0] - g[1, 1]) < 1e-8
+    assert abs(g[0, 1]) < 1e-8
+
+
+def test_ewald_shell_convergence():
+    # ewald.rs:972-988
+    a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
+    k = 2 * np.pi / 600.0
+    r = [15.0, 8.0, 3.0]
+    g5 = O.ewald(a1, a2, r, [0, 0, 0], k, 0, n_real=5)
+    g10 = O.ewald(a1, a2, r, [0, 0, 0], k, 0, n_real=10)
+    assert np.max(np.abs(g5 - g10)) < 0.05 * np.max(np.abs(g5))
+
+
+def test_ewald_real_space_is_damped_point_sum():
+    # real_space with one shell = erfc(eta|s|) G(s) summed over 9 images
+    a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
+    k = 2 * np.pi / 600.0
+    r = np.array([15.0, 8.0, 3.0])
+    eta = O.lattice_eta(a1, a2)
+    ref = np.zeros((3, 3), complex)
+    for m1 in (-1, 0, 1):
+        for m2 in (-1, 0, 1):
+            R = np.array([100.0 * m1, 100.0 * m2, 0.0])
+            s = r - R
+            ref += scipy.special.erfc(eta * np.linalg.norm(s)) * O.greens_point(s, [0, 0, 0], k)
+    got = O.ewald(a1, a2, r, [0, 0, 0], k, 1, n_real=1)
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 2e-6
+
+
+# ---- substrate (substrate.rs:160-197) ----------------------------------------
+
+def test_substrate_known_answers():
+    # substrate.rs:204-276
+    assert abs(O.substrate_factor(2.25, 1.0, False) - 1.25 / 3.25) < 1e-10
+    assert abs(O.substrate_factor(2.25, 1.0, True) - 0.2) < 1e-10
+    assert abs(O.substrate_factor(2.25, 2.25, True)) < 1e-12
+    assert abs(O.substrate_factor(2.0, 2.0, False)) < 1e-12
+    assert abs(O.substrate_factor(complex(-100, 5), 1.0, False)) > 0.95
+    # matches the product-side helper
+    from lumina_amd.solver import substrate_reflection_factor
+    for e in (2.25, complex(-10, 1.5), complex(3.1, 0.2)):
+        for ret in (True, False):
+            assert abs(substrate_reflection_factor(e, 1.33 ** 2, ret)
+                       - O.substrate_factor(e, 1.33 ** 2, ret)) < 1e-14
+
+
+# ---- invert_3x3 (assembly.rs:335-392) ---------------------------------------
+
+def test_invert3():
+    assert np.allclose(O.invert_3x3(np.eye(3).reshape(9)), np.eye(3).reshape(9), atol=1e-12)
+    a = 2.0 + 0.5j
+    inv = O.invert_3x3((a * np.eye(3)).reshape(9))
+    assert abs(inv[0] - 1 / a) < 1e-12 and abs(inv[4] - 1 / a) < 1e-12
+    r = np.random.default_rng(0)
+    M = r.standard_normal((3, 3)) + 1j * r.standard_normal((3, 3))
+    assert np.allclose(O.invert_3x3(M.reshape(9)).reshape(3, 3), np.linalg.inv(M), atol=1e-12)
+
+
+# ---- GMRES (iterative.rs:33-323) -------------------------------------------
+
+def test_gmres_identity():
+    n = 6
+    rhs = np.array([complex(i + 1, 0.5 * i) for i in range(n)])
+    x, info = O.solve_gmres_dense(np.eye(n), rhs, 1e-10, 100)
+    assert np.max(np.abs(x - rhs)) < 1e-8 and info["status"] == 0
+
+
+def test_gmres_complex_3x3():
+    A = np.array([[4 + 1j, -1 + 0.5j, 0], [-1 - 0.5j, 4, -1 + 0.2j], [0, -1 - 0.2j, 4 - 1j]])
+    b = np.array([1, 1j, -1 + 0.5j])
+    x, info = O.solve_gmres_dense(A, b, 1e-12, 100)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+
+
+def test_gmres_diagonal_known_solution():
+    n = 9
+    d = np.array([complex(2 + i, 0.3 * i) for i in range(n)])
+    x, _ = O.solve_gmres_dense(np.diag(d), d * (1 - 0.5j), 1e-12, 100)
+    assert np.max(np.abs(x - (1 - 0.5j))) < 1e-8
+
+
+def test_gmres_zero_matrix_fails():
+    x, info = O.solve_gmres_dense(np.zeros((3, 3)), np.array([1, 1j, 1 + 1j]), 1e-10, 50)
+    assert info["status"] == 5
+
+
+def test_gmres_restarts_match_lapack():
+    r = np.random.default_rng(1)
+    n = 120
+    A = np.eye(n) * 4 + (r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))) / np.sqrt(n)
+    b = r.standard_normal(n) + 1j * r.standard_normal(n)
+    x, info = O.solve_gmres_dense(A, b, 1e-12, 1000)
+    assert info["cycles"] >= 1
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(x) < 1e-10
+
+
+# ---- matvec, solve, cross sections -------------------------------------------
+
+def _small(use_fcd=True, sub=None, n_env=1.33, lam=520.0):
+    pos = I.discretise(I.Sphere(6.0), 2.0)
+    a = I.polarisability_block("Au", lam, 2.0, n_env)
+    return O.Problem(pos, a, I.wavenumber(lam, n_env), use_fcd, 2.0, substrate=sub)
+
+
+@pytest.mark.parametrize("use_fcd,sub", [(True, None), (False, None), (True, (-5.0, 0.2 + 0.05j))])
+def test_matvec_equals_assembled(use_fcd, sub):
+    P = _small(use_fcd, sub)
+    r = np.random.default_rng(2)
+    x = r.standard_normal(3 * P.n) + 1j * r.standard_normal(3 * P.n)
+    A = P.assemble()
+    assert np.max(np.abs(P.matvec(x) - A @ x)) / np.max(np.abs(A @ x)) < 1e-13
+
+
+def test_matvec_brute_force_python():
+    # a 4-dipole system assembled in plain Python from the golden-checked G
+    pos = np.array([[0, 0, 0], [2, 0, 0], [0, 3, 1], [5, 5, 5.0]])
+    a = I.isotropic_block(40 + 10j)
+    k = 2 * np.pi / 500
+    P = O.Problem(pos, a, k, use_fcd=False)
+    x = np.arange(12) * (1 + 0.5j) - 3
+    y = np.zeros(12, complex)
+    for i in range(4):
+        y[3 * i:3 * i + 3] = x[3 * i:3 * i + 3] / (40 + 10j)
+        for j in range(4):
+            if i != j:
+                y[3 * i:3 * i + 3] -= O.greens_point(pos[i], pos[j], k) @ x[3 * j:3 * j + 3]
+    assert np.max(np.abs(P.matvec(x) - y)) < 1e-13 * np.max(np.abs(y))
+
+
+def test_solve_matches_lapack_and_cross_sections():
+    P = _small()
+    e = P.incident()
+    cs, p, info = P.compute_cross_sections(1e-13, 1000)
+    A = P.assemble()
+    pl = np.linalg.solve(A, e)
+    assert np.linalg.norm(p - pl) / np.linalg.norm(pl) < 1e-11
+    csl = P.cross_sections(pl, e)
+    assert np.max(np.abs(cs - csl) / np.abs(csl)) < 1e-11
+    # direct formula for C_ext
+    k = P.k
+    assert abs(cs[0] - k * np.sum(np.imag(np.conj(e) * pl))) < 1e-10 * abs(cs[0])
+
+
+def test_single_dipole_rayleigh():
+    # cda_diagnostic.rs:119-167: one dipole, C_ext = k Im(alpha) |E0|^2
+    a = I.radiative_correction(I.clausius_mossotti(8.0, complex(-3.9, 2.6), 1.77), 0.016)
+    P = O.Problem(np.zeros((1, 3)), I.isotropic_block(a), 0.016, True, 2.0)
+    cs, p, info = P.compute_cross_sections(1e-12, 100)
+    assert abs(cs[0] - 0.016 * a.imag) < 1e-12 * abs(cs[0])
+    # radiative correction makes C_sca = (2/3) k^4 |alpha|^2 / ... consistent: C_abs + C_sca = C_ext
+    assert abs(cs[1] + cs[2] - cs[0]) < 1e-12 * abs(cs[0])
+
+
+def test_cross_section_energy_balance():
+    P = _small()
+    cs, p, _ = P.compute_cross_sections(1e-12, 1000)
+    assert cs[0] > 0 and cs[1] > 0 and cs[2] >= 0
+    assert abs(cs[0] - cs[1] - cs[2]) < 1e-9 * cs[0]
+
+
+def test_gmres_matvec_count_formula():
+    # matvecs = total_iters + 2 cycles (iterative.rs)
+    P = _small()
+    _, _, info = P.compute_cross_sections(1e-12, 1000)
+    assert info["matvecs"] == info["iterations"] + 2 * info["cycles"]
+
+
+def test_mutation_is_caught():
+    """A planted mistake in the tensor (b coefficient sign) fails the golden
+    check: the golden vectors are not a retyping of the oracle formula."""
+    cases = json.load(open(os.path.join(GOLD, "greens_point.json")))
+    c = cases[0]
+    G = _t(c["G"])
+    d = np.array(c["r1"]) - np.array(c["r2"])
+    k = c["k"]
+    R = np.linalg.norm(d)
+    kr = k * R
+    pf = k * k * np.exp(1j * kr) / (4 * np.pi * R)
+    a = 1 + (1j * kr - 1) / kr ** 2
+    b_wrong = (3 + 3j * kr - kr ** 2) / kr ** 2   # sign of the 3ikR term flipped
+    rh = d / R
+    Gw = pf * (a * np.eye(3) + b_wrong * np.outer(rh, rh))
+    assert np.max(np.abs(Gw - G)) / np.max(np.abs(G)) > 1e-6
