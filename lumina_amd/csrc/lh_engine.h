@@ -58,7 +58,7 @@ struct Problem {
 
 // Far-field tile choice (the autotuner's search space).
 struct Tiling {
-  int target_blocks = 4096;   // desired far-field work items per matvec
+  int target_blocks = 2048;   // far-field work items of the largest solve alone
   int min_chunk = 128;        // shortest source chunk
 };
 

@@ -177,22 +177,17 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     fcd.insert(fcd.end(), g.fcd.begin(), g.fcd.end());
   }
 
-  // far-field chunk length: the shortest multiple of 64 (>= min_chunk) whose
-  // item count does not exceed the target
+  // Far-field source chunk length (a multiple of 64): short enough that the
+  // largest solve alone yields about target_blocks work items, because the
+  // lock-step batch thins out as solves converge and the largest solves are
+  // usually the last ones running.
   long long maxn = 0;
   for (auto& p : probs_) maxn = std::max<long long>(maxn, geoms_[p.geom].n);
-  int chunk = (int)std::max<long long>(64, ((maxn + 63) / 64) * 64);
-  for (int c = std::max(64, t.min_chunk); c < chunk; c += 64) {
-    long long items = 0;
-    for (auto& p : probs_) {
-      const int n = geoms_[p.geom].n;
-      items += (long long)((n + 255) / 256) * far_chunks(n, c);
-    }
-    if (items <= t.target_blocks) {
-      chunk = c;
-      break;
-    }
-  }
+  const long long tiles_max = (maxn + 255) / 256;
+  const long long chunks_max =
+      std::max<long long>(1, (long long)t.target_blocks / std::max<long long>(tiles_max, 1));
+  const long long want = (maxn + chunks_max - 1) / chunks_max;
+  int chunk = (int)std::max<long long>(t.min_chunk, ((want + 63) / 64) * 64);
 
   sd_.assign(S, SolveDev{});
   far_.clear();

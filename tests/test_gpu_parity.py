@@ -152,36 +152,70 @@ def test_convergence_failure(backend):
     assert ei.value.info["iterations"] == 5
 
 
-def test_batch_matches_single(backend):
+def _batch_probs():
     pos = I.au_sphere_r10_d2()
     pos2 = I.discretise(I.Ellipsoid((8.0, 5.0, 4.0)), 2.0)
+    d = I.depol_factors((8.0, 5.0, 4.0))
     probs = []
-    for lam in (450.0, 520.0, 610.0):
+    for lam in (400.0, 450.0, 500.0):
         k = I.wavenumber(lam, 1.33)
         probs.append(dict(positions=pos, polarisability=I.polarisability_block("Au", lam, 2.0, 1.33),
                           k=k, use_fcd=True, cell_size=2.0))
-        d = I.depol_factors((8.0, 5.0, 4.0))
         probs.append(dict(positions=pos2,
                           polarisability=I.polarisability_block("Ag", lam, 2.0, 1.33, depol=d),
                           k=k, use_fcd=True, cell_size=2.0))
-    cs, infos, moms = batch_compute(backend, probs, 1e-12, 1000, precision=FP64, want_moments=True)
+    return probs
+
+
+def test_batch_matches_single_including_failures(backend):
+    """Six solves of two geometries in one batch: every solve that the
+    reference's GMRES(30) converges (tol 1e-12, 1000 iterations) matches the
+    oracle to 1e-10; the one it does not (Ag ellipsoid at 450 nm) is reported
+    as a convergence failure after the same 1000 iterations."""
+    probs = _batch_probs()
+    with pytest.raises(ConvergenceFailure) as ei:
+        batch_compute(backend, probs, 1e-12, 1000, precision=FP64)
+    cs, infos = ei.value.cross_sections, ei.value.infos
     for q, pr in enumerate(probs):
         P = O.Problem(pr["positions"], pr["polarisability"], pr["k"], True, 2.0)
-        cso, po, _ = P.compute_cross_sections(1e-12, 1000)
-        assert np.max(np.abs(cs[q] - cso) / np.abs(cso)) < 1e-10, (q, cs[q], cso)
+        cso, po, io = P.compute_cross_sections(1e-12, 1000)
+        print(q, infos[q]["iterations"], io["iterations"], infos[q]["status"], io["status"])
+        assert (infos[q]["status"] == 0) == (io["status"] == 0), q
+        if io["status"] == 0:
+            assert np.max(np.abs(cs[q] - cso) / np.abs(cso)) < 1e-10, (q, cs[q], cso)
+            assert abs(infos[q]["iterations"] - io["iterations"]) <= max(2, io["iterations"] // 20)
+        else:
+            assert infos[q]["iterations"] == 1000
 
 
 def test_batch_mixed(backend):
-    pos = I.au_sphere_r10_d2()
-    probs = []
-    for lam in (400.0, 500.0, 700.0):
-        k = I.wavenumber(lam, 1.33)
-        probs.append(dict(positions=pos, polarisability=I.polarisability_block("Au", lam, 2.0, 1.33),
-                          k=k, use_fcd=True, cell_size=2.0))
+    probs = [p for p in _batch_probs() if len(p["positions"]) == 515]
     cs, infos, _ = batch_compute(backend, probs, 1e-10, 1000, precision=MIXED)
     for q, pr in enumerate(probs):
-        cso, _, _ = O.Problem(pos, pr["polarisability"], pr["k"], True, 2.0).compute_cross_sections(1e-12, 1000)
-        assert abs(cs[q][0] - cso[0]) / abs(cso[0]) < 1e-6
+        cso, _, _ = O.Problem(pr["positions"], pr["polarisability"], pr["k"], True,
+                              2.0).compute_cross_sections(1e-12, 1000)
+        err = abs(cs[q][0] - cso[0]) / abs(cso[0])
+        print("batch mixed C_ext err", err, infos[q])
+        assert err < 1e-6
+        assert infos[q]["matvecs_fp32"] > 0
+
+
+def test_batch_resident_rerun(backend):
+    """A batch stays on the device: two runs (FP64, then mixed) of the same
+    batch give the FP64 answer and the mixed answer within 1e-6."""
+    from lumina_amd.solver import Batch
+    probs = [p for p in _batch_probs() if len(p["positions"]) == 515]
+    b = Batch(backend, probs)
+    b.run(1e-10, 1000, FP64)
+    cs64, i64, _ = b.results()
+    b.set_timing(True)
+    b.run(1e-10, 1000, MIXED)
+    cs32, i32, _ = b.results()
+    kt = b.kernel_times()
+    b.close()
+    assert np.max(np.abs(cs32[:, 0] - cs64[:, 0]) / np.abs(cs64[:, 0])) < 1e-6
+    assert kt["far32_launches"] > 0 and kt["far32_ms"] > 0
+    assert kt["far32_pairs"] == sum(i["matvecs_fp32"] * 515 * 514 for i in i32)
 
 
 def test_stored_matrix_session(backend):
