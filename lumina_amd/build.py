@@ -21,7 +21,7 @@ ARCH = os.environ.get("LUMINA_HIP_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-Wno-unused-command-line-argument", "-fgpu-flush-denormals-to-zero",
 ]
 
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -91,18 +92,27 @@ lh::Geometry make_geometry(const lh_pairsum_spec* s) {
   g.n = (int)s->n_dipoles;
   g.use_fcd = s->use_fcd ? 1 : 0;
   g.cell = s->cell_size;
-  // Far-field exclusion radius: 5% beyond the FCD radius 2 cell_size, so
-  // every FCD pair is in the near list whatever the rounding of |d|.
+  // Far-field exclusion radius: 5% beyond the FCD radius 2 cell_size, tested
+  // on FP32-rounded positions; every FCD pair is inside it as long as FP32
+  // rounding of the positions moves a displacement by far less than 5% of
+  // 2 cell_size, which the coordinate bound below guarantees.
   const double R = 2.0 * s->cell_size * 1.05;
-  g.rskip2 = g.use_fcd ? R * R : 0.0;
+  g.rskip2f = g.use_fcd ? (float)(R * R) : 0.0f;
   g.x.resize(g.n);
   g.y.resize(g.n);
   g.z.resize(g.n);
+  double amax = 0.0;
   for (int i = 0; i < g.n; ++i) {
     g.x[i] = s->positions[3 * i];
     g.y[i] = s->positions[3 * i + 1];
     g.z[i] = s->positions[3 * i + 2];
+    if (!std::isfinite(g.x[i]) || !std::isfinite(g.y[i]) || !std::isfinite(g.z[i]))
+      throw lh::Error(LH_INVALID_ARGUMENT, "non-finite dipole position");
+    amax = std::max({amax, std::fabs(g.x[i]), std::fabs(g.y[i]), std::fabs(g.z[i])});
   }
+  if (g.use_fcd && amax > 1e4 * s->cell_size)
+    throw lh::Error(LH_INVALID_ARGUMENT,
+                    "positions span more than 1e4 cell sizes from the origin; centre the object");
   lh::build_near_list(g);
   return g;
 }
@@ -444,6 +454,8 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
     LH_HIP(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
     bt->eng = std::make_unique<lh::Engine>(bt->stream);
     lh::Tiling t;
+    const char* sc = std::getenv("LUMINA_FP32_SINCOS");
+    t.hw_sincos = (sc && std::string(sc) == "hw") ? 1 : 0;
     bt->eng->build(geoms, probs, t);
     for (size_t q = 0; q < count; ++q) bt->eng->fill_incident((int)q, problems[q].incident);
     *out = bt.release();

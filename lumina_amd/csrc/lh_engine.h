@@ -29,21 +29,26 @@ struct DevMem {
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
-// One dipole geometry (positions + FCD near-pair list), shared by every
-// solve on it.
+// One dipole geometry (positions + near list), shared by every solve on it.
 struct Geometry {
   int n = 0;
   int use_fcd = 0;
   double cell = 0.0;
-  double rskip2 = 0.0;
+  float rskip2f = 0.0f;        // exclusion radius^2 on FP32 positions (0: no near list)
   std::vector<double> x, y, z;
-  std::vector<long long> rp;   // n+1, local entry ids
-  std::vector<int> row, col;
-  std::vector<unsigned char> fcd;
+  int kw = 0;                  // ELL width (largest near count of a dipole)
+  std::vector<int> ell_col;    // [kw][n] source index j
+  std::vector<int> ell_tid;    // [kw][n] displacement-table id, -1 = pad (row end)
+  std::vector<double> disp;    // [ntab][4] distinct r_i - r_j and the FCD flag
+  int ntab = 0;
+  long long nnz = 0;
 };
 
-// Builds the near list: every pair j != i with |r_i - r_j|^2 <= rskip2,
-// flagged FCD when sqrt(|r_i - r_j|^2) <= 2 cell_size (greens.rs:106).
+// Builds the near list: every pair j != i whose FP32-rounded positions
+// satisfy r2f_exact <= rskip2f, i.e. exactly the pairs the far-field kernels
+// skip. Each pair points to a table of distinct displacements r_i - r_j
+// (FP64), flagged FCD when sqrt(|r_i - r_j|^2) <= 2 cell_size, the
+// reference's own test (greens.rs:102-106).
 void build_near_list(Geometry& g);
 
 struct Problem {
@@ -56,10 +61,9 @@ struct Problem {
   lh_c64 delta{0.0, 0.0};
 };
 
-// Far-field tile choice (the autotuner's search space).
+// Far-field launch options (the autotuner's search space).
 struct Tiling {
-  int target_blocks = 2048;   // far-field work items of the largest solve alone
-  int min_chunk = 128;        // shortest source chunk
+  int hw_sincos = 0;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
 };
 
 struct SolveResult {
@@ -120,17 +124,14 @@ class Engine {
   std::vector<Geometry> geoms_;
   std::vector<Problem> probs_;
   std::vector<SolveDev> sd_;
-  std::vector<FarItem> far_;
+  std::vector<FarItem> far_free_, far_sub_;
   std::vector<VecItem> vit_;
-  bool any_sub_ = false, any_free_ = false;
-  long long t_stride_ = 0;
   int n_titems_ = 0;
+  Tiling tiling_;
 
-  DevMem d_px_, d_py_, d_pz_, d_rp_, d_row_, d_col_, d_fcd_, d_t_, d_ainv_, d_vec_, d_src64_,
-      d_src32_, d_part_, d_dot_, d_norm_, d_cross_, d_solves_, d_state_, d_counter_, d_far_,
-      d_vit_, d_tis_, d_tie_, d_tee_;
-  hipGraphExec_t step_graph_[LH_MMAX][2] = {};
-  bool graph_ready_[LH_MMAX][2] = {};
+  DevMem d_px_, d_py_, d_pz_, d_ell_col_, d_ell_tid_, d_disp_, d_tab_, d_ainv_, d_vec_,
+      d_src64_, d_src32_, d_part_, d_dot_, d_norm_, d_cross_, d_solves_, d_state_, d_counter_,
+      d_far_free_, d_far_sub_, d_vit_, d_tis_, d_tiu_;
 };
 
 }  // namespace lh

@@ -11,20 +11,19 @@
 namespace lh {
 
 // kernel launchers (lh_pairsum.hip / lh_krylov.hip)
-void launch_near_tensors(hipStream_t, int, const SolveDev*, const int*, const long long*,
-                         const long long*, const double*, const double*, const double*,
-                         const int*, const int*, const unsigned char*, double*, long long);
+void launch_near_tensors(hipStream_t, int, const SolveDev*, const int*, const int*,
+                         const double*, double*);
 void launch_prep64(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const double*, const double*, const double*, const double2*, int, Src64*);
 void launch_prep32(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const double*, const double*, const double*, const double2*, int, Src32*);
 void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const double*, const double*, const double*, const Src64*, double2*);
+                  const Src64*, double2*);
 void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const double*, const double*, const double*, const Src32*, double2*);
+                  const Src32*, double2*, bool);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
-                   const long long*, const int*, const double*, long long, const double2*,
-                   double2*, int, int, const double2*, int, double*);
+                   const int*, const int*, const double*, const double2*, double2*, int, int,
+                   const double2*, int, double*);
 void launch_vnorm(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                   const double2*, int, double*);
 void launch_scale(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
@@ -79,23 +78,50 @@ static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Near list: spatial hash with bins of side sqrt(rskip2); every pair within
-// the exclusion radius is found by scanning the 27 neighbouring bins.
+// Near list: spatial hash with bins of side sqrt(rskip2f); every pair within
+// the exclusion radius is found by scanning the 27 neighbouring bins. The
+// membership test is r2f_exact on FP32-rounded positions, the expression the
+// far-field kernels skip on.
 // ---------------------------------------------------------------------------
+namespace {
+struct DispKey {
+  uint64_t a, b, c;
+  bool operator==(const DispKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct DispHash {
+  size_t operator()(const DispKey& k) const {
+    return (size_t)(k.a * 0x9E3779B97F4A7C15ULL ^ (k.b + 0x7F4A7C15ULL) * 0xC2B2AE3D27D4EB4FULL ^
+                    (k.c + 0x165667B1ULL) * 0x27D4EB2F165667C5ULL);
+  }
+};
+}  // namespace
+
 void build_near_list(Geometry& g) {
   const int n = g.n;
-  g.rp.assign(n + 1, 0);
-  g.row.clear();
-  g.col.clear();
-  g.fcd.clear();
-  if (!g.use_fcd || n == 0) return;
-  const double h = std::sqrt(g.rskip2);
+  g.kw = 0;
+  g.ell_col.clear();
+  g.ell_tid.clear();
+  g.disp.clear();
+  g.ntab = 0;
+  g.nnz = 0;
+  if (!(g.rskip2f > 0.0f) || n == 0) return;
+  std::vector<float> xf(n), yf(n), zf(n);
+  for (int i = 0; i < n; ++i) {
+    xf[i] = (float)g.x[i];
+    yf[i] = (float)g.y[i];
+    zf[i] = (float)g.z[i];
+  }
   double mn[3] = {g.x[0], g.y[0], g.z[0]};
+  double amax = 0.0;
   for (int i = 0; i < n; ++i) {
     mn[0] = std::min(mn[0], g.x[i]);
     mn[1] = std::min(mn[1], g.y[i]);
     mn[2] = std::min(mn[2], g.z[i]);
+    amax = std::max({amax, std::fabs(g.x[i]), std::fabs(g.y[i]), std::fabs(g.z[i])});
   }
+  // bin side: the exclusion radius plus the largest change FP32 rounding of
+  // the positions can make to a displacement
+  const double h = std::sqrt((double)g.rskip2f) * (1.0 + 1e-6) + 4.0 * amax * 1.2e-7 + 1e-12;
   auto key = [](long long a, long long b, long long c) {
     return (a * 73856093LL) ^ (b * 19349663LL) ^ (c * 83492791LL);
   };
@@ -108,9 +134,9 @@ void build_near_list(Geometry& g) {
     bz[i] = (long long)std::floor((g.z[i] - mn[2]) / h);
     bins[key(bx[i], by[i], bz[i])].push_back(i);
   }
-  std::vector<int> cand;
+  std::vector<std::vector<int>> rows(n);
   for (int i = 0; i < n; ++i) {
-    cand.clear();
+    auto& cand = rows[i];
     for (int a = -1; a <= 1; ++a)
       for (int b = -1; b <= 1; ++b)
         for (int c = -1; c <= 1; ++c) {
@@ -119,20 +145,41 @@ void build_near_list(Geometry& g) {
           for (int j : it->second) {
             if (j == i) continue;
             if (bx[j] != bx[i] + a || by[j] != by[i] + b || bz[j] != bz[i] + c) continue;
-            const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
-            if (r2_exact(dx, dy, dz) <= g.rskip2) cand.push_back(j);
+            if (r2f_exact(xf[i] - xf[j], yf[i] - yf[j], zf[i] - zf[j]) <= g.rskip2f)
+              cand.push_back(j);
           }
         }
     std::sort(cand.begin(), cand.end());
-    for (int j : cand) {
+    g.kw = std::max<int>(g.kw, (int)cand.size());
+    g.nnz += (long long)cand.size();
+  }
+  g.ell_col.assign((size_t)g.kw * n, 0);
+  g.ell_tid.assign((size_t)g.kw * n, -1);
+  std::unordered_map<DispKey, int, DispHash> tab;
+  for (int i = 0; i < n; ++i) {
+    for (size_t kk = 0; kk < rows[i].size(); ++kk) {
+      const int j = rows[i][kk];
       const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
-      // greens.rs:102-106: dist = sqrt(dx^2 + dy^2 + dz^2); FCD unless dist > 2 cell
-      const double dist = std::sqrt(r2_exact(dx, dy, dz));
-      g.row.push_back(i);
-      g.col.push_back(j);
-      g.fcd.push_back((dist > 2.0 * g.cell) ? 0 : 1);
+      DispKey dk{__builtin_bit_cast(uint64_t, dx), __builtin_bit_cast(uint64_t, dy),
+                 __builtin_bit_cast(uint64_t, dz)};
+      auto it = tab.find(dk);
+      int u;
+      if (it == tab.end()) {
+        u = g.ntab++;
+        tab.emplace(dk, u);
+        // greens.rs:102-106: dist = sqrt(dx^2 + dy^2 + dz^2); FCD unless dist > 2 cell
+        const double dist = std::sqrt(r2_exact(dx, dy, dz));
+        const bool fcd = g.use_fcd && !(dist > 2.0 * g.cell);
+        g.disp.push_back(dx);
+        g.disp.push_back(dy);
+        g.disp.push_back(dz);
+        g.disp.push_back(fcd ? 1.0 : 0.0);
+      } else {
+        u = it->second;
+      }
+      g.ell_col[kk * (size_t)n + i] = j;
+      g.ell_tid[kk * (size_t)n + i] = u;
     }
-    g.rp[i + 1] = (long long)g.col.size();
   }
 }
 
@@ -140,64 +187,40 @@ void build_near_list(Geometry& g) {
 // Batch construction.
 // ---------------------------------------------------------------------------
 
-Engine::~Engine() {
-  clear_events();
-  for (int j = 0; j < LH_MMAX; ++j)
-    for (int f = 0; f < 2; ++f)
-      if (graph_ready_[j][f]) (void)hipGraphExecDestroy(step_graph_[j][f]);
-}
-
-static int far_chunks(int n, int chunk) { return (n + chunk - 1) / chunk; }
+Engine::~Engine() { clear_events(); }
 
 void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t) {
   geoms_ = std::move(geoms);
   probs_ = std::move(probs);
+  tiling_ = t;
   const int S = (int)probs_.size();
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
 
-  // geometry arrays
-  std::vector<double> px, py, pz;
-  std::vector<long long> rp;
-  std::vector<int> row, col;
-  std::vector<unsigned char> fcd;
-  std::vector<int> g_pos(geoms_.size()), g_rp(geoms_.size());
-  std::vector<long long> g_e0(geoms_.size());
+  std::vector<double> px, py, pz, disp;
+  std::vector<int> ecol, etid;
+  std::vector<int> g_pos(geoms_.size());
+  std::vector<long long> g_ell(geoms_.size()), g_disp(geoms_.size());
   for (size_t gi = 0; gi < geoms_.size(); ++gi) {
     Geometry& g = geoms_[gi];
     g_pos[gi] = (int)px.size();
-    g_rp[gi] = (int)rp.size();
-    const long long base = (long long)col.size();
-    g_e0[gi] = base;
+    g_ell[gi] = (long long)ecol.size();
+    g_disp[gi] = (long long)disp.size() / 4;
     px.insert(px.end(), g.x.begin(), g.x.end());
     py.insert(py.end(), g.y.begin(), g.y.end());
     pz.insert(pz.end(), g.z.begin(), g.z.end());
-    for (long long v : g.rp) rp.push_back(base + v);
-    row.insert(row.end(), g.row.begin(), g.row.end());
-    col.insert(col.end(), g.col.begin(), g.col.end());
-    fcd.insert(fcd.end(), g.fcd.begin(), g.fcd.end());
+    ecol.insert(ecol.end(), g.ell_col.begin(), g.ell_col.end());
+    etid.insert(etid.end(), g.ell_tid.begin(), g.ell_tid.end());
+    disp.insert(disp.end(), g.disp.begin(), g.disp.end());
   }
 
-  // Far-field source chunk length (a multiple of 64): short enough that the
-  // largest solve alone yields about target_blocks work items, because the
-  // lock-step batch thins out as solves converge and the largest solves are
-  // usually the last ones running.
-  long long maxn = 0;
-  for (auto& p : probs_) maxn = std::max<long long>(maxn, geoms_[p.geom].n);
-  const long long tiles_max = (maxn + 255) / 256;
-  const long long chunks_max =
-      std::max<long long>(1, (long long)t.target_blocks / std::max<long long>(tiles_max, 1));
-  const long long want = (maxn + chunks_max - 1) / chunks_max;
-  int chunk = (int)std::max<long long>(t.min_chunk, ((want + 63) / 64) * 64);
-
   sd_.assign(S, SolveDev{});
-  far_.clear();
+  far_free_.clear();
+  far_sub_.clear();
   vit_.clear();
-  long long vec_off = 0, src_off = 0, part_off = 0, t_off = 0, ainv_off = 0;
+  long long vec_off = 0, src_off = 0, part_off = 0, tab_off = 0, ainv_off = 0;
   int vi_off = 0;
   std::vector<lh_c64> ainv_all;
-  std::vector<int> tis;
-  std::vector<long long> tie, tee;
-  any_sub_ = any_free_ = false;
+  std::vector<int> tis, tiu;
   for (int s = 0; s < S; ++s) {
     const Problem& p = probs_[s];
     const Geometry& g = geoms_[p.geom];
@@ -205,74 +228,65 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     d.n = g.n;
     d.m = std::min(LH_MMAX, 3 * g.n);
     d.pos_off = g_pos[p.geom];
-    d.rp_off = g_rp[p.geom];
-    d.e0 = g_e0[p.geom];
-    d.t_off = t_off;
+    d.kw = g.kw;
+    d.ell_off = g_ell[p.geom];
+    d.tab_off = tab_off;
+    d.dtab_off = g_disp[p.geom];
+    d.ntab = g.ntab;
+    d.nt = (g.n + LH_TILE - 1) / LH_TILE;
     d.ainv_off = ainv_off;
     d.ainv_uniform = p.ainv_uniform;
     d.has_sub = p.has_sub;
     d.vec_off = vec_off;
     d.src_off = src_off;
     d.part_off = part_off;
-    d.nsc = far_chunks(g.n, chunk);
     d.vi_off = vi_off;
     d.nvi = (g.n + LH_VB - 1) / LH_VB;
     d.max_iter = 0;
+    d.rskip2f = g.rskip2f;
     d.k = p.k;
     d.inv_k = 1.0 / p.k;
     d.c0 = p.k * p.k / (4.0 * M_PI);
     d.z2 = 2.0 * p.z_interface;
     d.dre = p.delta.re;
     d.dim = p.delta.im;
-    d.rskip2 = g.rskip2;
     d.cell = g.cell;
     d.tol = 0.0;
-    if (p.has_sub) any_sub_ = true;
-    else any_free_ = true;
-    for (int t0 = 0; t0 < g.n; t0 += 256)
-      for (int c = 0; c < d.nsc; ++c) {
-        FarItem f;
-        f.s = s;
-        f.t0 = t0;
-        f.c0 = c * chunk;
-        f.c1 = std::min(g.n, (c + 1) * chunk);
-        f.chunk = c;
-        f.pad = 0;
-        far_.push_back(f);
+    for (int ti = 0; ti < d.nt; ++ti)
+      for (int tj = 0; tj < d.nt; ++tj) {
+        FarItem f{s, ti * LH_TILE, tj * LH_TILE, std::min(g.n, (tj + 1) * LH_TILE)};
+        if (p.has_sub) far_sub_.push_back(f);
+        else far_free_.push_back(f);
       }
     for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
-    const long long nnz = (long long)g.col.size();
-    for (long long e = 0; e < nnz; e += 256) {
+    for (int u = 0; u < g.ntab; u += 256) {
       tis.push_back(s);
-      tie.push_back(d.e0 + e);
-      tee.push_back(d.e0 + std::min(nnz, e + 256));
+      tiu.push_back(u);
     }
     ainv_all.insert(ainv_all.end(), p.ainv.begin(), p.ainv.end());
     ainv_off += (long long)p.ainv.size();
-    t_off += nnz;
+    tab_off += g.ntab;
     vec_off += (long long)LH_NSLOT * 3 * g.n;
     src_off += g.n;
-    part_off += (long long)d.nsc * 3 * g.n;
+    part_off += (long long)d.nt * 3 * g.n;
     vi_off += d.nvi;
   }
-  t_stride_ = std::max<long long>(t_off, 1);
   n_titems_ = (int)tis.size();
 
   hipStream_t st = stream_;
   upload(d_px_, px, st);
   upload(d_py_, py, st);
   upload(d_pz_, pz, st);
-  upload(d_rp_, rp, st);
-  upload(d_row_, row, st);
-  upload(d_col_, col, st);
-  upload(d_fcd_, fcd, st);
+  upload(d_ell_col_, ecol, st);
+  upload(d_ell_tid_, etid, st);
+  upload(d_disp_, disp, st);
   upload(d_ainv_, ainv_all, st);
-  upload(d_far_, far_, st);
+  upload(d_far_free_, far_free_, st);
+  upload(d_far_sub_, far_sub_, st);
   upload(d_vit_, vit_, st);
   upload(d_tis_, tis, st);
-  upload(d_tie_, tie, st);
-  upload(d_tee_, tee, st);
-  d_t_.alloc(sizeof(double) * 12 * (size_t)t_stride_);
+  upload(d_tiu_, tiu, st);
+  d_tab_.alloc(sizeof(double) * 12 * (size_t)std::max<long long>(tab_off, 1));
   d_vec_.alloc(sizeof(double2) * (size_t)std::max<long long>(vec_off, 1));
   LH_HIP(hipMemsetAsync(d_vec_.p, 0, d_vec_.bytes, st));
   d_src64_.alloc(sizeof(Src64) * (size_t)src_off);
@@ -319,10 +333,7 @@ void Engine::set_problem(int s, const Problem& p) {
 
 void Engine::compute_near_tensors() {
   launch_near_tensors(stream_, n_titems_, d_solves_.as<SolveDev>(), d_tis_.as<int>(),
-                      d_tie_.as<long long>(), d_tee_.as<long long>(), d_px_.as<double>(),
-                      d_py_.as<double>(), d_pz_.as<double>(), d_row_.as<int>(),
-                      d_col_.as<int>(), d_fcd_.as<unsigned char>(), d_t_.as<double>(),
-                      t_stride_);
+                      d_tiu_.as<int>(), d_disp_.as<double>(), d_tab_.as<double>());
   LH_HIP(hipGetLastError());
 }
 
@@ -371,7 +382,8 @@ double Engine::pairs_per_matvec(int s) const {
 
 void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   hipStream_t st = stream_;
-  const int nv = (int)vit_.size(), nf = (int)far_.size();
+  const int nv = (int)vit_.size();
+  const int nfree = (int)far_free_.size(), nsub = (int)far_sub_.size();
   const VecItem* vi = d_vit_.as<VecItem>();
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
@@ -385,33 +397,25 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    if (any_free_)
-      launch_far32(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
-                   d_py_.as<double>(), d_pz_.as<double>(), d_src32_.as<Src32>(),
-                   d_part_.as<double2>());
-    if (any_sub_)
-      launch_far32(st, true, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
-                   d_py_.as<double>(), d_pz_.as<double>(), d_src32_.as<Src32>(),
-                   d_part_.as<double2>());
+    launch_far32(st, false, nfree, d_far_free_.as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
+                 d_part_.as<double2>(), tiling_.hw_sincos != 0);
+    launch_far32(st, true, nsub, d_far_sub_.as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
+                 d_part_.as<double2>(), tiling_.hw_sincos != 0);
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    if (any_free_)
-      launch_far64(st, false, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
-                   d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
-                   d_part_.as<double2>());
-    if (any_sub_)
-      launch_far64(st, true, nf, d_far_.as<FarItem>(), sd, gs, which, d_px_.as<double>(),
-                   d_py_.as<double>(), d_pz_.as<double>(), d_src64_.as<Src64>(),
-                   d_part_.as<double2>());
+    launch_far64(st, false, nfree, d_far_free_.as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
+                 d_part_.as<double2>());
+    launch_far64(st, true, nsub, d_far_sub_.as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
+                 d_part_.as<double2>());
   }
   if (timing_) {
     LH_HIP(hipEventRecord(ev.b, st));
     evs_.push_back(ev);
   }
-  launch_reduce(st, nv, vi, sd, gs, which, d_rp_.as<long long>(), d_col_.as<int>(),
-                d_t_.as<double>(), t_stride_, d_ainv_.as<double2>(), vec, in_slot, out_slot,
+  launch_reduce(st, nv, vi, sd, gs, which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
+                d_tab_.as<double>(), d_ainv_.as<double2>(), vec, in_slot, out_slot,
                 d_part_.as<double2>(), mode, d_norm_.as<double>());
 }
 

@@ -5,54 +5,56 @@
 //
 // (assembly.rs:209-318 matvec_on_the_fly). The pair sum is split in two:
 //
-//  * far field (k_far64 / k_far32): every pair outside the exclusion radius
-//    (and j != i) evaluates the point Green's tensor on the fly
-//    (greens.rs:34-71) in the y += A x + B d (d . x) form -- no tensor is
-//    materialised. One target dipole per lane; the source records stream
-//    through the scalar cache (uniform loads into SGPRs), so a source is
-//    fetched once per wave and consumed by 64 targets without LDS traffic.
-//    Work items are (solve, 256-target tile, source chunk); each writes its
-//    own partial sum, so no atomics are needed and results are
-//    reproducible.
+//  * far field: every pair outside the exclusion radius (and j != i)
+//    evaluates the point Green's tensor (greens.rs:34-71) on the fly in the
+//    y += A x + B d (d . x) form -- no tensor is materialised. One target
+//    dipole per lane; source records stream through the scalar cache (every
+//    lane of a wave reads the same record), so a source costs no LDS traffic.
+//    Work items are 256 x 256 tiles of (targets, sources); each writes the
+//    row sums of its targets to the partial slice of its source tile (no
+//    atomics, reproducible results). With a substrate (k_far_row<R, true>,
+//    assembly.rs:272-295) the image tensor at the mirrored source is applied
+//    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
+//    |d|^2 < 1e-20.
 //  * near field (k_near_tensors + k_reduce): pairs inside the exclusion
-//    radius use tensors precomputed once per solve in FP64: the FCD 27-node
-//    cell average (greens.rs:94-167) where dist <= 2 cell_size, the point
-//    tensor otherwise. k_reduce adds alpha^-1 x, the far partials and the
-//    near tensors and writes y (or the residual rhs - y and its norm).
+//    radius use FP64 tensors precomputed once per solve for each distinct
+//    displacement of the geometry: the FCD 27-node cell average
+//    (greens.rs:94-167) where dist <= 2 cell_size, the point tensor
+//    otherwise. k_reduce adds alpha^-1 x, the far partials and the near
+//    tensors (an ELL list of table ids) and writes y, or the residual
+//    rhs - y and its norm.
 //
-// The image term (assembly.rs:272-295, Fresnel branch) is a second point
-// tensor per pair at the mirrored source, applied to (-D, -D, +D) o x_j,
-// for every pair including j == i, skipped when |d|^2 < 1e-20.
+// The exclusion test |d|^2 <= (2.1 cell)^2 is evaluated on FP32-rounded
+// positions with r2f_exact in every kernel and in the host near-list
+// builder, so the far field and the near list cover every pair exactly once.
+#include <type_traits>
+
 #include "lh_device.h"
 
 namespace lh {
 
 // ---------------------------------------------------------------------------
-// Near-field tensors (once per solve).
+// Near-field tensors (once per solve), one thread per distinct displacement.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_near_tensors(
-    const SolveDev* __restrict__ solves, const int* __restrict__ item_s,
-    const long long* __restrict__ item_e, int nitems, const double* __restrict__ px,
-    const double* __restrict__ py, const double* __restrict__ pz,
-    const int* __restrict__ near_row, const int* __restrict__ near_col,
-    const unsigned char* __restrict__ near_fcd, double* __restrict__ near_t, long long t_stride, const long long* __restrict__ e_end) {
+__global__ __launch_bounds__(256) void k_near_tensors(const SolveDev* __restrict__ solves,
+                                                     const int* __restrict__ item_s,
+                                                     const int* __restrict__ item_u,
+                                                     int nitems,
+                                                     const double* __restrict__ disp,
+                                                     double* __restrict__ tab) {
   const int b = blockIdx.x;
   if (b >= nitems) return;
-  const int s = item_s[b];
-  const SolveDev S = solves[s];
-  const long long e = item_e[b] + threadIdx.x;
-  if (e >= e_end[b]) return;
-  const int i = near_row[e];
-  const int j = near_col[e];
-  const double xi = px[S.pos_off + i], yi = py[S.pos_off + i], zi = pz[S.pos_off + i];
-  const double xj = px[S.pos_off + j], yj = py[S.pos_off + j], zj = pz[S.pos_off + j];
+  const SolveDev S = solves[item_s[b]];
+  const int u = item_u[b] + threadIdx.x;
+  if (u >= S.ntab) return;
+  const double* d = disp + 4 * (S.dtab_off + u);
+  const double dx = d[0], dy = d[1], dz = d[2];
   double t[12];
-  if (near_fcd[e]) {
+  if (d[3] != 0.0) {
     // greens.rs:120-164: nodes r_j + (d/2) n, n in {-sqrt(3/5), 0, sqrt(3/5)}^3,
     // weights w_x w_y w_z from {5/9, 8/9, 5/9}, normalised by the total weight.
-    const double cs = S.cell;
-    const double half_d = cs / 2.0;
+    const double half_d = S.cell / 2.0;
     const double nodes[3] = {-sqrt(3.0 / 5.0), 0.0, sqrt(3.0 / 5.0)};
     const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
     double acc[12];
@@ -62,13 +64,13 @@ __global__ __launch_bounds__(256) void k_near_tensors(
     for (int ix = 0; ix < 3; ++ix)
       for (int iy = 0; iy < 3; ++iy)
         for (int iz = 0; iz < 3; ++iz) {
-          const double dx = xi - (xj + half_d * nodes[ix]);
-          const double dy = yi - (yj + half_d * nodes[iy]);
-          const double dz = zi - (zj + half_d * nodes[iz]);
-          if (r2_exact(dx, dy, dz) < 1e-20) continue;
+          const double ex = dx - half_d * nodes[ix];
+          const double ey = dy - half_d * nodes[iy];
+          const double ez = dz - half_d * nodes[iz];
+          if (r2_exact(ex, ey, ez) < 1e-20) continue;
           const double w = wts[ix] * wts[iy] * wts[iz];
           double g[12];
-          g6_point64(dx, dy, dz, S.k, S.inv_k, S.c0, g);
+          g6_point64(ex, ey, ez, S.k, S.inv_k, S.c0, g);
 #pragma unroll
           for (int q = 0; q < 12; ++q) acc[q] += w * g[q];
           tw += w;
@@ -77,14 +79,11 @@ __global__ __launch_bounds__(256) void k_near_tensors(
 #pragma unroll
     for (int q = 0; q < 12; ++q) t[q] = acc[q] * inv_w;
   } else {
-    g6_point64(xi - xj, yi - yj, zi - zj, S.k, S.inv_k, S.c0, t);
+    g6_point64(dx, dy, dz, S.k, S.inv_k, S.c0, t);
   }
-  const long long te = S.t_off + (e - S.e0);
+  double* o = tab + 12 * (S.tab_off + u);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    near_t[(2 * q) * t_stride + te] = t[2 * q];
-    near_t[(2 * q + 1) * t_stride + te] = t[2 * q + 1];
-  }
+  for (int q = 0; q < 12; ++q) o[q] = t[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -102,9 +101,18 @@ __device__ __forceinline__ bool takes_part(const GState* st, int s, int which) {
 }
 
 // ---------------------------------------------------------------------------
-// Source records for one matvec: position + input vector (+ image-scaled
+// Source records for one matvec: positions + input vector (+ image-scaled
 // input). x is read from the Krylov slot `slot` of each solve.
 // ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void set_pos(Src64& r, double x, double y, double z) {
+  r.x = x; r.y = y; r.z = z;
+  r.xf = (float)x; r.yf = (float)y; r.zf = (float)z; r.pf = 0.0f;
+  r.pad = 0.0;
+}
+__device__ __forceinline__ void set_pos(Src32& r, double x, double y, double z) {
+  r.x = (float)x; r.y = (float)y; r.z = (float)z; r.pf = 0.0f;
+}
 
 template <typename SRC>
 __global__ __launch_bounds__(LH_VB) void k_prep(const VecItem* __restrict__ items,
@@ -138,9 +146,7 @@ __global__ __launch_bounds__(LH_VB) void k_prep(const VecItem* __restrict__ item
     w[2 * c + 1] = mr * v[2 * c + 1] + mi * v[2 * c];
   }
   SRC r;
-  r.x = px[S.pos_off + j];
-  r.y = py[S.pos_off + j];
-  r.z = pz[S.pos_off + j];
+  set_pos(r, px[S.pos_off + j], py[S.pos_off + j], pz[S.pos_off + j]);
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     r.v[q] = static_cast<typename SRC::value_type>(v[q]);
@@ -150,80 +156,73 @@ __global__ __launch_bounds__(LH_VB) void k_prep(const VecItem* __restrict__ item
 }
 
 // ---------------------------------------------------------------------------
-// Far field, FP64.
+// Pair arithmetic, FP64 and FP32.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void far_pair64(double dx, double dy, double dz, double r2,
-                                           double scale, const double* __restrict__ v,
-                                           double k, double inv_k, double c0,
-                                           double acc[6]) {
-  double Ar, Ai, Br, Bi;
+template <typename R> struct SrcOf;
+template <> struct SrcOf<double> { typedef Src64 type; };
+template <> struct SrcOf<float> { typedef Src32 type; };
+
+__device__ __forceinline__ float fx(const Src64& r) { return r.xf; }
+__device__ __forceinline__ float fy(const Src64& r) { return r.yf; }
+__device__ __forceinline__ float fz(const Src64& r) { return r.zf; }
+__device__ __forceinline__ float fx(const Src32& r) { return r.x; }
+__device__ __forceinline__ float fy(const Src32& r) { return r.y; }
+__device__ __forceinline__ float fz(const Src32& r) { return r.z; }
+
+// G(d) = A I + B d d^T (greens.rs:34-71), scaled by `scale`.
+template <bool HW = false>
+__device__ __forceinline__ void coeffs(double r2, double scale, double k, double inv_k, double c0,
+                                       double& Ar, double& Ai, double& Br, double& Bi) {
   g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi);
   Ar *= scale; Ai *= scale; Br *= scale; Bi *= scale;
-  // d . x_j
-  const double dr = dx * v[0] + dy * v[2] + dz * v[4];
-  const double di = dx * v[1] + dy * v[3] + dz * v[5];
-  const double wr = Br * dr - Bi * di;
-  const double wi = Br * di + Bi * dr;
-  acc[0] += Ar * v[0] - Ai * v[1] + dx * wr;
-  acc[1] += Ar * v[1] + Ai * v[0] + dx * wi;
-  acc[2] += Ar * v[2] - Ai * v[3] + dy * wr;
-  acc[3] += Ar * v[3] + Ai * v[2] + dy * wi;
-  acc[4] += Ar * v[4] - Ai * v[5] + dz * wr;
-  acc[5] += Ar * v[5] + Ai * v[4] + dz * wi;
+}
+// HW = true: sin/cos from the hardware v_sin_f32/v_cos_f32 (argument in
+// revolutions, kr / 2 pi); otherwise sincos_f32.
+template <bool HW = false>
+__device__ __forceinline__ void coeffs(float r2, float scale, float k, float inv_k, float c0,
+                                       float& Ar, float& Ai, float& Br, float& Bi) {
+  const float inv_r = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * inv_r;
+  float s, c;
+  if constexpr (HW) {
+    const float rev = (k * r) * 0.15915494309189535f;
+    s = __builtin_amdgcn_sinf(rev);
+    c = __builtin_amdgcn_cosf(rev);
+  } else {
+    sincos_f32(k * r, s, c);
+  }
+  const float u = inv_r * inv_k;       // 1 / (k r)
+  const float u2 = u * u;
+  const float pf = c0 * inv_r * scale; // k^2 / (4 pi r)
+  const float pr = pf * c, pi = pf * s;
+  const float ar = 1.0f - u2, ai = u;
+  const float br = 3.0f * u2 - 1.0f, bi = -3.0f * u;
+  const float ir2 = inv_r * inv_r;
+  Ar = pr * ar - pi * ai;
+  Ai = pr * ai + pi * ar;
+  Br = (pr * br - pi * bi) * ir2;
+  Bi = (pr * bi + pi * br) * ir2;
 }
 
-template <bool SUB>
-__global__ __launch_bounds__(256) void k_far64(const FarItem* __restrict__ items,
-                                               const SolveDev* __restrict__ solves,
-                                               const GState* __restrict__ st, int which,
-                                               const double* __restrict__ px,
-                                               const double* __restrict__ py,
-                                               const double* __restrict__ pz,
-                                               const Src64* __restrict__ src,
-                                               double2* __restrict__ part) {
-  const FarItem it = items[blockIdx.x];
-  if (!takes_part(st, it.s, which)) return;
-  const SolveDev S = solves[it.s];
-  if (SUB != (S.has_sub != 0)) return;
-  const int i = it.t0 + threadIdx.x;
-  const bool valid = i < S.n;
-  const int ii = valid ? i : 0;
-  const double xi = px[S.pos_off + ii], yi = py[S.pos_off + ii], zi = pz[S.pos_off + ii];
-  const double k = S.k, inv_k = S.inv_k, c0 = S.c0, rskip2 = S.rskip2, z2 = S.z2;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  const Src64* __restrict__ sp = src + S.src_off;
-  for (int j = it.c0; j < it.c1; ++j) {
-    const Src64& r = sp[j];
-    const double dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
-    const double d2 = r2_exact(dx, dy, dz);
-    const bool skip = d2 <= rskip2;
-    far_pair64(dx, dy, dz, skip ? 1.0 : d2, skip ? 0.0 : 1.0, r.v, k, inv_k, c0, acc);
-    if (SUB) {
-      const double ez = zi - (z2 - r.z);
-      const double e2 = r2_exact(dx, dy, ez);
-      const bool eskip = e2 < 1e-20;
-      far_pair64(dx, dy, ez, eskip ? 1.0 : e2, eskip ? 0.0 : 1.0, r.w, k, inv_k, c0, acc);
-    }
-  }
-  if (valid) {
-    double2* o = part + S.part_off + (long long)it.chunk * 3 * S.n + 3LL * i;
-    o[0] = make_double2(acc[0], acc[1]);
-    o[1] = make_double2(acc[2], acc[3]);
-    o[2] = make_double2(acc[4], acc[5]);
-  }
+// out += (A I + B d d^T) v, as fused multiply-add chains into the
+// accumulators (3 FMA per output component).
+template <typename R>
+__device__ __forceinline__ void apply(R dx, R dy, R dz, R Ar, R Ai, R Br, R Bi, const R* v,
+                                      R out[6]) {
+  const R dr = fma(dz, v[4], fma(dy, v[2], dx * v[0]));
+  const R di = fma(dz, v[5], fma(dy, v[3], dx * v[1]));
+  const R wr = fma(-Bi, di, Br * dr);
+  const R wi = fma(Bi, dr, Br * di);
+  out[0] = fma(dx, wr, fma(-Ai, v[1], fma(Ar, v[0], out[0])));
+  out[1] = fma(dx, wi, fma(Ai, v[0], fma(Ar, v[1], out[1])));
+  out[2] = fma(dy, wr, fma(-Ai, v[3], fma(Ar, v[2], out[2])));
+  out[3] = fma(dy, wi, fma(Ai, v[2], fma(Ar, v[3], out[3])));
+  out[4] = fma(dz, wr, fma(-Ai, v[5], fma(Ar, v[4], out[4])));
+  out[5] = fma(dz, wi, fma(Ai, v[4], fma(Ar, v[5], out[5])));
 }
 
-// ---------------------------------------------------------------------------
-// Far field, FP32 evaluation with compensated accumulation. The exclusion
-// test and the displacement are FP64 (the same test as the FP64 kernel and
-// the near list); G and the products are FP32. Terms are summed in FP32
-// over runs of LH_RUN sources, and the run sums are folded into a float-float
-// (hi, lo) accumulator by an error-free TwoSum.
-// ---------------------------------------------------------------------------
-
-#define LH_RUN 32
-
+// Error-free TwoSum of hi + v into (hi, lo).
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float v) {
   const float s = hi + v;
   const float bb = s - hi;
@@ -232,97 +231,92 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float v) {
   lo += err;
 }
 
-__device__ __forceinline__ void far_pair32(float dx, float dy, float dz, float r2, float scale,
-                                           const float* __restrict__ v, float k, float inv_k,
-                                           float c0, float acc[6]) {
-  const float inv_r = rsqrtf(r2);
-  const float r = r2 * inv_r;
-  float s, c;
-  sincosf(k * r, &s, &c);
-  const float u = inv_r * inv_k;
-  const float u2 = u * u;
-  const float pf = c0 * inv_r * scale;
-  const float pr = pf * c, pi = pf * s;
-  const float ar = 1.0f - u2, ai = u;
-  const float br = 3.0f * u2 - 1.0f, bi = -3.0f * u;
-  const float ir2 = inv_r * inv_r;
-  const float Ar = pr * ar - pi * ai;
-  const float Ai = pr * ai + pi * ar;
-  const float Br = (pr * br - pi * bi) * ir2;
-  const float Bi = (pr * bi + pi * br) * ir2;
-  const float dr = dx * v[0] + dy * v[2] + dz * v[4];
-  const float di = dx * v[1] + dy * v[3] + dz * v[5];
-  const float wr = Br * dr - Bi * di;
-  const float wi = Br * di + Bi * dr;
-  acc[0] += Ar * v[0] - Ai * v[1] + dx * wr;
-  acc[1] += Ar * v[1] + Ai * v[0] + dx * wi;
-  acc[2] += Ar * v[2] - Ai * v[3] + dy * wr;
-  acc[3] += Ar * v[3] + Ai * v[2] + dy * wi;
-  acc[4] += Ar * v[4] - Ai * v[5] + dz * wr;
-  acc[5] += Ar * v[5] + Ai * v[4] + dz * wi;
-}
+// Row sums: FP64 accumulates directly; FP32 sums runs of LH_RUN sources in
+// FP32 and folds each run into a float-float (hi, lo) pair.
+#define LH_RUN 32
 
-template <bool SUB>
-__global__ __launch_bounds__(256) void k_far32(const FarItem* __restrict__ items,
-                                               const SolveDev* __restrict__ solves,
-                                               const GState* __restrict__ st, int which,
-                                               const double* __restrict__ px,
-                                               const double* __restrict__ py,
-                                               const double* __restrict__ pz,
-                                               const Src32* __restrict__ src,
-                                               double2* __restrict__ part) {
+template <typename R>
+struct RowAcc {
+  R hi[6], lo[6];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) hi[q] = lo[q] = R(0);
+  }
+  __device__ __forceinline__ void fold(const R acc[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if constexpr (std::is_same<R, float>::value) two_sum(hi[q], lo[q], acc[q]);
+      else hi[q] += acc[q];
+    }
+  }
+  __device__ __forceinline__ double get(int q) const { return (double)hi[q] + (double)lo[q]; }
+};
+
+// ---------------------------------------------------------------------------
+// Far field: every ordered tile pair, row sums; SUB adds the Fresnel image.
+// ---------------------------------------------------------------------------
+
+template <typename R, bool SUB, bool HW>
+__global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ items,
+                                                 const SolveDev* __restrict__ solves,
+                                                 const GState* __restrict__ st, int which,
+                                                 const typename SrcOf<R>::type* __restrict__ src,
+                                                 double2* __restrict__ part) {
+  typedef typename SrcOf<R>::type SRC;
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
-  if (SUB != (S.has_sub != 0)) return;
   const int i = it.t0 + threadIdx.x;
   const bool valid = i < S.n;
-  const int ii = valid ? i : 0;
-  const double xi = px[S.pos_off + ii], yi = py[S.pos_off + ii], zi = pz[S.pos_off + ii];
-  const float k = (float)S.k, inv_k = (float)S.inv_k, c0 = (float)S.c0;
-  const double rskip2 = S.rskip2, z2 = S.z2;
-  float hi[6] = {0, 0, 0, 0, 0, 0}, lo[6] = {0, 0, 0, 0, 0, 0};
-  const Src32* __restrict__ sp = src + S.src_off;
+  const SRC* __restrict__ sp = src + S.src_off;
+  const SRC& me = sp[valid ? i : 0];
+  const float xif = fx(me), yif = fy(me), zif = fz(me);
+  const R xi = me.x, yi = me.y, zi = me.z;
+  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0, z2 = (R)S.z2;
+  const float rskip2f = S.rskip2f;
+  RowAcc<R> row;
+  row.zero();
   for (int j0 = it.c0; j0 < it.c1; j0 += LH_RUN) {
     const int j1 = min(j0 + LH_RUN, it.c1);
-    float acc[6] = {0, 0, 0, 0, 0, 0};
+    R acc[6] = {0, 0, 0, 0, 0, 0};
     for (int j = j0; j < j1; ++j) {
-      const Src32& r = sp[j];
-      const double dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
-      const double d2 = r2_exact(dx, dy, dz);
-      const bool skip = d2 <= rskip2;
-      far_pair32((float)dx, (float)dy, (float)dz, skip ? 1.0f : (float)d2, skip ? 0.0f : 1.0f,
-                 r.v, k, inv_k, c0, acc);
+      const SRC& r = sp[j];
+      const float d2f = r2f_exact(xif - fx(r), yif - fy(r), zif - fz(r));
+      const bool skip = (d2f <= rskip2f) || (j == i);
+      const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
+      // FP32: the exclusion test's |d|^2 is the value's |d|^2
+      const R r2 = std::is_same<R, float>::value ? (R)d2f : dx * dx + dy * dy + dz * dz;
+      R Ar, Ai, Br, Bi;
+      coeffs<HW>(skip ? R(1) : r2, skip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+      apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, r.v, acc);
       if (SUB) {
-        const double ez = zi - (z2 - r.z);
-        const double e2 = r2_exact(dx, dy, ez);
-        const bool eskip = e2 < 1e-20;
-        far_pair32((float)dx, (float)dy, (float)ez, eskip ? 1.0f : (float)e2,
-                   eskip ? 0.0f : 1.0f, r.w, k, inv_k, c0, acc);
+        // image of source j: (x_j, y_j, 2 z0 - z_j)
+        const R ez = zi - (z2 - r.z);
+        const R e2 = dx * dx + dy * dy + ez * ez;
+        const bool eskip = e2 < R(1e-20);
+        coeffs<HW>(eskip ? R(1) : e2, eskip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+        apply<R>(dx, dy, ez, Ar, Ai, Br, Bi, r.w, acc);
       }
     }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) two_sum(hi[q], lo[q], acc[q]);
+    row.fold(acc);
   }
   if (valid) {
-    double2* o = part + S.part_off + (long long)it.chunk * 3 * S.n + 3LL * i;
+    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      o[c] = make_double2((double)hi[2 * c] + (double)lo[2 * c],
-                          (double)hi[2 * c + 1] + (double)lo[2 * c + 1]);
+    for (int c = 0; c < 3; ++c) o[c] = make_double2(row.get(2 * c), row.get(2 * c + 1));
   }
 }
 
 // ---------------------------------------------------------------------------
-// Reduction: y_i = alpha_i^-1 x_i - sum_chunks part - sum_near T_ij x_j.
+// Reduction: y_i = alpha_i^-1 x_i - sum_tiles part - sum_near T_ij x_j.
 // mode 0 writes y to slot `out_slot`; mode 1 writes r = rhs - y to the
 // residual slot and a per-item partial of |r|^2 to norm_part.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(LH_VB) void k_reduce(
     const VecItem* __restrict__ items, const SolveDev* __restrict__ solves,
-    const GState* __restrict__ st, int which, const long long* __restrict__ near_rp,
-    const int* __restrict__ near_col, const double* __restrict__ near_t, long long t_stride,
+    const GState* __restrict__ st, int which, const int* __restrict__ ell_col,
+    const int* __restrict__ ell_tid, const double* __restrict__ tab,
     const double2* __restrict__ ainv, double2* __restrict__ vec, int in_slot, int out_slot,
     const double2* __restrict__ part, int mode, double* __restrict__ norm_part) {
   __shared__ double lds[4];
@@ -355,7 +349,7 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
       yr[a] = sr;
       yi[a] = si;
     }
-    for (int ch = 0; ch < S.nsc; ++ch) {
+    for (int ch = 0; ch < S.nt; ++ch) {
       const double2* p = part + S.part_off + (long long)ch * n3 + 3LL * i;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -364,13 +358,12 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
         yi[c] -= q.y;
       }
     }
-    const long long e_lo = near_rp[S.rp_off + i], e_hi = near_rp[S.rp_off + i + 1];
-    for (long long e = e_lo; e < e_hi; ++e) {
-      const int j = near_col[e];
-      const long long te = S.t_off + (e - S.e0);
-      double t[12];
-#pragma unroll
-      for (int q = 0; q < 12; ++q) t[q] = near_t[q * t_stride + te];
+    for (int kk = 0; kk < S.kw; ++kk) {
+      const long long e = S.ell_off + (long long)kk * S.n + i;
+      const int u = ell_tid[e];
+      if (u < 0) break;  // pads sit at the end of a row
+      const int j = ell_col[e];
+      const double* t = tab + 12 * (S.tab_off + u);
       double vr[3], vi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -420,14 +413,10 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
 // ---------------------------------------------------------------------------
 
 void launch_near_tensors(hipStream_t st, int nitems, const SolveDev* solves, const int* item_s,
-                         const long long* item_e, const long long* e_end, const double* px,
-                         const double* py, const double* pz, const int* near_row,
-                         const int* near_col, const unsigned char* near_fcd, double* near_t,
-                         long long t_stride) {
+                         const int* item_u, const double* disp, double* tab) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(k_near_tensors, dim3(nitems), dim3(256), 0, st, solves, item_s, item_e,
-                     nitems, px, py, pz, near_row, near_col, near_fcd, near_t, t_stride,
-                     e_end);
+  hipLaunchKernelGGL(k_near_tensors, dim3(nitems), dim3(256), 0, st, solves, item_s, item_u,
+                     nitems, disp, tab);
 }
 
 void launch_prep64(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
@@ -445,36 +434,41 @@ void launch_prep32(hipStream_t st, int nitems, const VecItem* items, const Solve
 }
 
 void launch_far64(hipStream_t st, bool sub, int nitems, const FarItem* items,
-                  const SolveDev* solves, const GState* gs, int which, const double* px,
-                  const double* py, const double* pz, const Src64* src, double2* part) {
+                  const SolveDev* solves, const GState* gs, int which, const Src64* src,
+                  double2* part) {
   if (nitems <= 0) return;
   if (sub)
-    hipLaunchKernelGGL(k_far64<true>, dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
-                       px, py, pz, src, part);
+    hipLaunchKernelGGL((k_far_row<double, true, false>), dim3(nitems), dim3(256), 0, st, items, solves,
+                       gs, which, src, part);
   else
-    hipLaunchKernelGGL(k_far64<false>, dim3(nitems), dim3(256), 0, st, items, solves, gs,
-                       which, px, py, pz, src, part);
+    hipLaunchKernelGGL((k_far_row<double, false, false>), dim3(nitems), dim3(256), 0, st, items, solves,
+                       gs, which, src, part);
 }
 
 void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
-                  const SolveDev* solves, const GState* gs, int which, const double* px,
-                  const double* py, const double* pz, const Src32* src, double2* part) {
+                  const SolveDev* solves, const GState* gs, int which, const Src32* src,
+                  double2* part, bool hw_sincos) {
   if (nitems <= 0) return;
-  if (sub)
-    hipLaunchKernelGGL(k_far32<true>, dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
-                       px, py, pz, src, part);
+  if (sub && hw_sincos)
+    hipLaunchKernelGGL((k_far_row<float, true, true>), dim3(nitems), dim3(256), 0, st, items,
+                       solves, gs, which, src, part);
+  else if (sub)
+    hipLaunchKernelGGL((k_far_row<float, true, false>), dim3(nitems), dim3(256), 0, st, items,
+                       solves, gs, which, src, part);
+  else if (hw_sincos)
+    hipLaunchKernelGGL((k_far_row<float, false, true>), dim3(nitems), dim3(256), 0, st, items,
+                       solves, gs, which, src, part);
   else
-    hipLaunchKernelGGL(k_far32<false>, dim3(nitems), dim3(256), 0, st, items, solves, gs,
-                       which, px, py, pz, src, part);
+    hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(256), 0, st, items,
+                       solves, gs, which, src, part);
 }
 
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
-                   const GState* gs, int which, const long long* near_rp, const int* near_col,
-                   const double* near_t, long long t_stride, const double2* ainv, double2* vec,
-                   int in_slot, int out_slot, const double2* part, int mode, double* norm_part) {
+                   const GState* gs, int which, const int* ell_col, const int* ell_tid,
+                   const double* tab, const double2* ainv, double2* vec, int in_slot,
+                   int out_slot, const double2* part, int mode, double* norm_part) {
   hipLaunchKernelGGL(k_reduce, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, which,
-                     near_rp, near_col, near_t, t_stride, ainv, vec, in_slot, out_slot, part,
-                     mode, norm_part);
+                     ell_col, ell_tid, tab, ainv, vec, in_slot, out_slot, part, mode, norm_part);
 }
 
 }  // namespace lh

@@ -2,18 +2,23 @@
 //
 // A batch holds S solves (one solve = one dipole set at one wavelength and
 // one incident field). Solves that share a geometry share its positions and
-// FCD near-pair list; everything else is per solve.
+// near list; everything else is per solve.
 //
 // HBM layout (all arrays device-resident for the life of the batch):
-//   pos_x/pos_y/pos_z [sum over geometries of n]       FP64 SoA positions
-//   near_rp           [sum over geometries of (n+1)]   CSR row pointers (global entry ids)
-//   near_col          [sum of nnz]                     source index j (local to the geometry)
-//   near_fcd          [sum of nnz]                     1: FCD 27-node average, 0: point G
-//   near_t            [6][sum over solves of nnz]      FP64 complex near tensors (xx yy zz xy xz yz)
-//   ainv              [sum over solves of 9*n or 9]    FP64 complex alpha^-1 blocks
-//   vec               [sum over solves of NSLOT*3n]    FP64 complex Krylov storage, slot-major
-//   src64 / src32     [sum over solves of n]           per-matvec source records
-//   part              [sum over solves of nsc*3n]      far-field partial sums per source chunk
+//   pos_x/pos_y/pos_z   [sum over geometries of n]    FP64 SoA positions
+//   posf_x/posf_y/posf_z[sum over geometries of n]    the same rounded to FP32
+//                                                     (exclusion test of every kernel)
+//   ell_col / ell_tid   [sum over geometries of kw*n] ELL near list: source j and
+//                                                     displacement-table id (-1 = pad),
+//                                                     slot-major so lanes read consecutively
+//   near_tab            [sum over solves of 12*U]     FP64 near tensors (xx yy zz xy xz yz,
+//                                                     re/im) per unique displacement
+//   tab_disp            [sum over geometries of 4*U]  unique displacement (dx dy dz, fcd flag)
+//   ainv                [sum over solves of 9*n or 9] FP64 complex alpha^-1 blocks
+//   vec                 [sum over solves of NSLOT*3n] FP64 complex Krylov storage, slot-major
+//   src64 / src32       [sum over solves of n]        per-matvec source records
+//   part                [sum over solves of nt*3n]    far-field partial sums, one slice per
+//                                                     256-dipole tile
 #pragma once
 #include <stdint.h>
 
@@ -26,44 +31,46 @@
 #define LH_NSLOT (LH_MMAX + 6)
 
 #define LH_VB 256              // threads per vector-op block; 256 dipoles = 768 complex per item
+#define LH_TILE 256            // far-field tile: 256 targets x 256 sources
 #define LH_NDOT 32             // stride of the per-item dot partials
 
 struct SolveDev {
   int n;            // dipoles
   int m;            // restart length min(30, 3n)
-  int pos_off;      // first dipole in pos_x/y/z
-  int rp_off;       // first row pointer in near_rp (n+1 entries)
-  long long e0;     // near_rp[rp_off] (global id of the first near entry)
-  long long t_off;  // first tensor of this solve in near_t (per component)
+  int pos_off;      // first dipole in the position arrays
+  int kw;           // ELL width of the near list
+  long long ell_off;   // first ELL entry (kw*n entries, slot-major)
+  long long tab_off;   // first near tensor of this solve (12 doubles each)
+  long long dtab_off;  // first unique displacement of the geometry (4 doubles each)
+  int ntab;            // unique displacements of the geometry
+  int nt;              // 256-dipole tiles = far partial slices
   long long ainv_off;
   int ainv_uniform;
   int has_sub;
   long long vec_off;   // complex offset of slot 0
   long long src_off;   // first source record
-  long long part_off;  // complex offset of chunk 0 partials
-  int nsc;             // source chunks (partials per target)
+  long long part_off;  // complex offset of tile 0 partials
   int vi_off;          // first vector item of this solve
   int nvi;             // vector items of this solve
   int max_iter;
+  float rskip2f;       // far-field exclusion radius^2, FP32 (0 without FCD)
   double k;
   double inv_k;
   double c0;           // k^2 / (4 pi)
   double z2;           // 2 * z_interface
   double dre, dim;     // Fresnel factor Delta
-  double rskip2;       // far-field exclusion radius^2 (0 without FCD)
   double cell;         // FCD cell size
   double tol;
 };
 
-// Far-field work item: targets [t0, t0 + tile) of solve s against sources
-// [c0, c1), written to partial chunk `chunk`.
+// Far-field work item: targets [t0, t0 + 256) against sources [c0, c1) of
+// solve s. Row sums go to partial slice c0/256, column sums (symmetric
+// kernels, off-diagonal tiles) to slice t0/256.
 struct FarItem {
   int s;
   int t0;
   int c0;
   int c1;
-  int chunk;
-  int pad;
 };
 
 // Vector work item: dipoles [t0, t0 + LH_VB) of solve s (3 complex each).
@@ -100,20 +107,21 @@ struct GState {
   double H_re[(LH_MMAX + 1) * LH_MMAX], H_im[(LH_MMAX + 1) * LH_MMAX];
 };
 
-// FP64 source record: position and the (optionally image-scaled) input
-// vector of one dipole, read by every lane of a wave through the scalar
-// cache.
+// FP64 source record of one dipole for one matvec, read by every lane of a
+// wave through the scalar cache.
 struct Src64 {
   typedef double value_type;
-  double x, y, z, pad;
+  double x, y, z;       // position
+  float xf, yf, zf, pf; // position rounded to FP32 (exclusion test)
   double v[6];          // x_j (re, im) x 3
   double w[6];          // (-D, -D, +D) o x_j for the image term
+  double pad;
 };
 
+// FP32 source record.
 struct Src32 {
   typedef float value_type;
-  double x, y, z;       // FP64: exclusion test and displacement
+  float x, y, z, pf;    // position rounded to FP32
   float v[6];
   float w[6];
-  float pad[2];
 };
