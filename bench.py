@@ -128,6 +128,7 @@ def run_batch(args):
         kt = batch.kernel_times()
         for key in kt_tot:
             kt_tot[key] += kt[key]
+        blocks = (kt["far64_block"], kt["far32_block"], kt["tune_ms"])
         log(f"[bench] step {s}: {dt:.2f} s, iters mean "
             f"{np.mean([i['iterations'] for i in infos]):.1f}, max {max(i['iterations'] for i in infos)}")
     total = sum(times)
@@ -164,6 +165,9 @@ def run_batch(args):
     if not args.no_fp64_plain:
         sub = probs[args.wavelengths // 2::args.wavelengths][: args.plain_count]
         t_pl, ev_pl = 0.0, 0.0
+        # the far-field block size the batch's tuning chose for FP64, so the
+        # sessions below do not each time the candidates inside the solve
+        os.environ.setdefault("LUMINA_FAR_BLOCK", str(blocks[0]))
         for p in sub:
             s_ = PairSumSession(be, p["positions"], p["polarisability"], p["k"], True,
                                 p["cell_size"], precision=FP64)
@@ -186,10 +190,14 @@ def run_batch(args):
     far_n = kt_tot["far32_launches"] if prec == MIXED else kt_tot["far64_launches"]
     achieved = FLOPS_PER_FAR_PAIR * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else 0.0
     peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
-    roof = {"bound": "valu", "kernel": "k_far32<false>" if prec == MIXED else "k_far64<false>",
+    hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
+    kname = (f"k_far_row<float, false, {str(hw).lower()}>" if prec == MIXED
+             else "k_far_row<double, false, false>")
+    roof = {"bound": "valu", "kernel": kname,
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": None, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
             "flops_per_pair": FLOPS_PER_FAR_PAIR,
+            "targets_per_item": blocks[1] if prec == MIXED else blocks[0],
             "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0}
 
     line = {

@@ -209,6 +209,9 @@ typedef struct {
   uint64_t far64_launches, far32_launches;
   double far64_pairs, far32_pairs;
   double run_ms;          /* wall time of the last lh_batch_run */
+  int32_t far64_block;    /* far-field targets per work item in use (FP64, FP32) */
+  int32_t far32_block;
+  double tune_ms;         /* time spent choosing them (0 when fixed) */
 } lh_kernel_times;
 int lh_batch_set_timing(lh_batch* batch, int enable);
 int lh_batch_kernel_times(lh_batch* batch, lh_kernel_times* out);

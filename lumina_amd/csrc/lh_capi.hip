@@ -129,6 +129,25 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
   return p;
 }
 
+// Launch options. LUMINA_FP32_SINCOS=sw selects the software sin/cos of the
+// FP32 far field (default: the hardware instructions where their range
+// allows); LUMINA_GRAPH=0 issues every cycle's kernels from the host instead
+// of replaying a captured graph; LUMINA_FAR_BLOCK=64|128|256 fixes the
+// far-field targets per work item (default: timed on the batch).
+lh::Tiling tiling_from_env() {
+  lh::Tiling t;
+  const char* sc = std::getenv("LUMINA_FP32_SINCOS");
+  if (sc && std::string(sc) == "sw") t.hw_sincos = 0;
+  const char* gr = std::getenv("LUMINA_GRAPH");
+  if (gr && std::string(gr) == "0") t.graph = 0;
+  const char* bl = std::getenv("LUMINA_FAR_BLOCK");
+  if (bl) {
+    const int b = std::atoi(bl);
+    if (b == 64 || b == 128 || b == 256) t.block[0] = t.block[1] = b;
+  }
+  return t;
+}
+
 void fill_info(const lh::SolveResult& r, lh_solve_info* info) {
   if (!info) return;
   info->iterations = (uint64_t)r.iterations;
@@ -300,8 +319,7 @@ int lh_pairsum_create(lh_backend* b, const lh_pairsum_spec* spec, lh_pairsum** o
     std::vector<lh::Problem> probs;
     probs.push_back(make_problem(spec, 0));
     s->eng = std::make_unique<lh::Engine>(s->stream);
-    lh::Tiling t;
-    s->eng->build(geoms, probs, t);
+    s->eng->build(geoms, probs, tiling_from_env());
     *out = s.release();
     return (int)LH_OK;
   });
@@ -341,7 +359,7 @@ int lh_pairsum_solve(lh_pairsum* s, const lh_c64* rhs, double tol, size_t max_it
     if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
     LH_HIP(hipSetDevice(s->b->device));
     s->eng->upload_slot(0, LH_SLOT_RHS, rhs);
-    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, false);
+    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, true);
     s->eng->download_slot(0, LH_SLOT_X, x);
     const auto r = s->eng->results();
     fill_info(r[0], info);
@@ -387,7 +405,7 @@ int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, dou
     if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
     LH_HIP(hipSetDevice(s->b->device));
     s->eng->fill_incident(0, *inc);
-    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, false);
+    s->eng->solve(tol, (long long)max_iter, s->precision == LH_PREC_MIXED, true);
     const auto r = s->eng->results();
     fill_info(r[0], info);
     const auto cs = s->eng->cross_sections({inc->amplitude});
@@ -453,10 +471,7 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
     }
     LH_HIP(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
     bt->eng = std::make_unique<lh::Engine>(bt->stream);
-    lh::Tiling t;
-    const char* sc = std::getenv("LUMINA_FP32_SINCOS");
-    t.hw_sincos = (sc && std::string(sc) == "hw") ? 1 : 0;
-    bt->eng->build(geoms, probs, t);
+    bt->eng->build(geoms, probs, tiling_from_env());
     for (size_t q = 0; q < count; ++q) bt->eng->fill_incident((int)q, problems[q].incident);
     *out = bt.release();
     return (int)LH_OK;
@@ -533,6 +548,9 @@ int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
       out->far64_pairs += p * (res[q].matvecs - res[q].matvecs32);
     }
     out->run_ms = bt->eng->last_run_ms;
+    out->far64_block = bt->eng->block(0);
+    out->far32_block = bt->eng->block(1);
+    out->tune_ms = bt->eng->tune_ms;
     return (int)LH_OK;
   });
 }

@@ -61,9 +61,12 @@ struct Problem {
   lh_c64 delta{0.0, 0.0};
 };
 
-// Far-field launch options (the autotuner's search space).
+// Far-field launch options. block[] is the autotuner's search space.
 struct Tiling {
-  int hw_sincos = 0;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
+  int hw_sincos = 1;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
+  int graph = 1;              // replay the restart cycle as a captured HIP graph
+  int block[2] = {0, 0};      // far-field targets per work item [FP64, FP32]:
+                              // 64, 128 or 256; 0 = timed on the batch at first use
 };
 
 struct SolveResult {
@@ -106,6 +109,8 @@ class Engine {
   // {far64_ms, far32_ms, far64_launches, far32_launches}.
   void far_times(double out[4]);
   double last_run_ms = 0.0;
+  double tune_ms = 0.0;
+  int block(int fp32) const { return tiling_.block[fp32]; }
 
  private:
   struct EvPair {
@@ -117,6 +122,13 @@ class Engine {
   void clear_events();
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
   void arnoldi_step(int j, bool fp32);
+  void enqueue_cycle(bool mixed, bool early_exit);
+  hipGraphExec_t cycle_graph_[2] = {nullptr, nullptr};   // [fp64, mixed]
+  void drop_graphs();
+  // far work items of precision p for the block size tiling_.block[p]
+  void build_far_items(int p);
+  // times one all-solve matvec per candidate block size, keeps the fastest
+  void autotune(int p);
   int read_counter();
   void upload_solves();
 
@@ -124,14 +136,15 @@ class Engine {
   std::vector<Geometry> geoms_;
   std::vector<Problem> probs_;
   std::vector<SolveDev> sd_;
-  std::vector<FarItem> far_free_, far_sub_;
+  std::vector<FarItem> far_free_[2], far_sub_[2];   // [FP64, FP32]
   std::vector<VecItem> vit_;
   int n_titems_ = 0;
   Tiling tiling_;
 
   DevMem d_px_, d_py_, d_pz_, d_ell_col_, d_ell_tid_, d_disp_, d_tab_, d_ainv_, d_vec_,
       d_src64_, d_src32_, d_part_, d_dot_, d_norm_, d_cross_, d_solves_, d_state_, d_counter_,
-      d_far_free_, d_far_sub_, d_vit_, d_tis_, d_tiu_;
+      d_vit_, d_tis_, d_tiu_;
+  DevMem d_far_free_[2], d_far_sub_[2];
 };
 
 }  // namespace lh

@@ -4,6 +4,7 @@
 #include "lh_device.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -18,9 +19,9 @@ void launch_prep64(hipStream_t, int, const VecItem*, const SolveDev*, const GSta
 void launch_prep32(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const double*, const double*, const double*, const double2*, int, Src32*);
 void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const Src64*, double2*);
+                  const Src64*, double2*, int);
 void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const Src32*, double2*, bool);
+                  const Src32*, double2*, bool, int);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
                    const double2*, int, double*);
@@ -70,9 +71,12 @@ void DevMem::alloc(size_t b) {
   }
 }
 
+// Copies v to device memory m, keeping m's allocation (and address) when it
+// is already large enough -- captured HIP graphs hold these addresses.
 template <class T>
 static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
-  m.alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+  const size_t need = std::max<size_t>(v.size() * sizeof(T), 16);
+  if (m.p == nullptr || m.bytes < need) m.alloc(need);
   if (!v.empty())
     LH_HIP(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
@@ -187,12 +191,36 @@ void build_near_list(Geometry& g) {
 // Batch construction.
 // ---------------------------------------------------------------------------
 
-Engine::~Engine() { clear_events(); }
+Engine::~Engine() {
+  clear_events();
+  for (auto& g : cycle_graph_)
+    if (g) (void)hipGraphExecDestroy(g);
+}
 
 void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t) {
   geoms_ = std::move(geoms);
   probs_ = std::move(probs);
   tiling_ = t;
+  // The hardware v_sin/v_cos take the phase in revolutions and are valid
+  // for |x| <= 256 revolutions; the largest far-field phase of a solve is
+  // k times its geometry's bounding-box diagonal (its image pairs included:
+  // the image of a source is at most twice as far).
+  double max_rev = 0.0;
+  for (const auto& p : probs_) {
+    const Geometry& g = geoms_[p.geom];
+    if (g.n == 0) continue;
+    double lo[3] = {g.x[0], g.y[0], g.z[0]}, hi[3] = {g.x[0], g.y[0], g.z[0]};
+    for (int i = 0; i < g.n; ++i) {
+      lo[0] = std::min(lo[0], g.x[i]); hi[0] = std::max(hi[0], g.x[i]);
+      lo[1] = std::min(lo[1], g.y[i]); hi[1] = std::max(hi[1], g.y[i]);
+      lo[2] = std::min(lo[2], g.z[i]); hi[2] = std::max(hi[2], g.z[i]);
+    }
+    double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                            (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    if (p.has_sub) diag = 2.0 * diag + 2.0 * std::fabs(p.z_interface) + 2.0 * std::max(std::fabs(lo[2]), std::fabs(hi[2]));
+    max_rev = std::max(max_rev, p.k * diag / (2.0 * M_PI));
+  }
+  if (max_rev > 200.0) tiling_.hw_sincos = 0;
   const int S = (int)probs_.size();
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
 
@@ -214,8 +242,6 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   }
 
   sd_.assign(S, SolveDev{});
-  far_free_.clear();
-  far_sub_.clear();
   vit_.clear();
   long long vec_off = 0, src_off = 0, part_off = 0, tab_off = 0, ainv_off = 0;
   int vi_off = 0;
@@ -252,12 +278,6 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     d.dim = p.delta.im;
     d.cell = g.cell;
     d.tol = 0.0;
-    for (int ti = 0; ti < d.nt; ++ti)
-      for (int tj = 0; tj < d.nt; ++tj) {
-        FarItem f{s, ti * LH_TILE, tj * LH_TILE, std::min(g.n, (tj + 1) * LH_TILE)};
-        if (p.has_sub) far_sub_.push_back(f);
-        else far_free_.push_back(f);
-      }
     for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
     for (int u = 0; u < g.ntab; u += 256) {
       tis.push_back(s);
@@ -281,8 +301,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   upload(d_ell_tid_, etid, st);
   upload(d_disp_, disp, st);
   upload(d_ainv_, ainv_all, st);
-  upload(d_far_free_, far_free_, st);
-  upload(d_far_sub_, far_sub_, st);
+  for (int q = 0; q < 2; ++q) build_far_items(q);
   upload(d_vit_, vit_, st);
   upload(d_tis_, tis, st);
   upload(d_tiu_, tiu, st);
@@ -301,6 +320,70 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   upload_solves();
   compute_near_tensors();
   LH_HIP(hipStreamSynchronize(st));
+}
+
+void Engine::build_far_items(int p) {
+  far_free_[p].clear();
+  far_sub_[p].clear();
+  // before tuning, any block size serves (the far items are rebuilt then)
+  const int tb = tiling_.block[p] > 0 ? tiling_.block[p] : LH_TILE;
+  for (size_t s = 0; s < sd_.size(); ++s) {
+    const SolveDev& d = sd_[s];
+    for (int ti = 0; ti < (d.n + tb - 1) / tb; ++ti)
+      for (int tj = 0; tj < d.nt; ++tj) {
+        FarItem f{(int)s, ti * tb, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
+        if (d.has_sub) far_sub_[p].push_back(f);
+        else far_free_[p].push_back(f);
+      }
+  }
+  drop_graphs();
+  upload(d_far_free_[p], far_free_[p], stream_);
+  upload(d_far_sub_[p], far_sub_[p], stream_);
+}
+
+void Engine::drop_graphs() {
+  for (auto& g : cycle_graph_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void Engine::autotune(int p) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const bool was_timing = timing_;
+  timing_ = false;
+  hipEvent_t a, b;
+  LH_HIP(hipEventCreate(&a));
+  LH_HIP(hipEventCreate(&b));
+  int best = LH_TILE;
+  float best_ms = 0.0f;
+  for (int cand : {64, 128, 256}) {
+    tiling_.block[p] = cand;
+    build_far_items(p);
+    matvec(3, LH_SLOT_E, LH_SLOT_W, 0, p != 0);   // warm-up
+    float tot = 0.0f;
+    for (int rep = 0; rep < 2; ++rep) {
+      LH_HIP(hipEventRecord(a, stream_));
+      matvec(3, LH_SLOT_E, LH_SLOT_W, 0, p != 0);
+      LH_HIP(hipEventRecord(b, stream_));
+      LH_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      LH_HIP(hipEventElapsedTime(&ms, a, b));
+      tot += ms;
+    }
+    if (cand == 64 || tot < best_ms) {
+      best = cand;
+      best_ms = tot;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  timing_ = was_timing;
+  tiling_.block[p] = best;
+  build_far_items(p);
+  LH_HIP(hipStreamSynchronize(stream_));
+  tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void Engine::upload_solves() {
@@ -383,7 +466,9 @@ double Engine::pairs_per_matvec(int s) const {
 void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   hipStream_t st = stream_;
   const int nv = (int)vit_.size();
-  const int nfree = (int)far_free_.size(), nsub = (int)far_sub_.size();
+  const int p = fp32 ? 1 : 0;
+  const int nfree = (int)far_free_[p].size(), nsub = (int)far_sub_[p].size();
+  const int blk = tiling_.block[p] > 0 ? tiling_.block[p] : LH_TILE;
   const VecItem* vi = d_vit_.as<VecItem>();
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
@@ -397,18 +482,18 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    launch_far32(st, false, nfree, d_far_free_.as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
-                 d_part_.as<double2>(), tiling_.hw_sincos != 0);
-    launch_far32(st, true, nsub, d_far_sub_.as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
-                 d_part_.as<double2>(), tiling_.hw_sincos != 0);
+    launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
+                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+    launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
+                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    launch_far64(st, false, nfree, d_far_free_.as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
-                 d_part_.as<double2>());
-    launch_far64(st, true, nsub, d_far_sub_.as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
-                 d_part_.as<double2>());
+    launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
+                 d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
+                 d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
   }
   if (timing_) {
     LH_HIP(hipEventRecord(ev.b, st));
@@ -484,6 +569,8 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   }
   upload_solves();
   sdd = d_solves_.as<SolveDev>();
+  if (tiling_.block[0] == 0) autotune(0);
+  if (mixed && tiling_.block[1] == 0) autotune(1);
   // x = 0, r = rhs - A 0 = rhs (the matvec of the zero vector is exactly 0)
   launch_copy_slot(st, nv, vi, sdd, vec, -1, LH_SLOT_X);
   launch_copy_slot(st, nv, vi, sdd, vec, LH_SLOT_RHS, LH_SLOT_R);
@@ -495,22 +582,56 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   // every cycle performs at least one Arnoldi step of every running solve,
   // so at most max_iter + 1 cycles can start
   long long cycles = 0;
+  const bool graph = use_graph && tiling_.graph && !timing_;
+  const int gi = mixed ? 1 : 0;
+  if (graph && !cycle_graph_[gi]) {
+    // one restart cycle, every kernel of it, captured once and replayed;
+    // solves that leave the Arnoldi loop early skip the remaining steps
+    hipGraph_t g = nullptr;
+    LH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    enqueue_cycle(mixed, false);
+    LH_HIP(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LH_HIP(e);
+    cycle_graph_[gi] = ex;
+  }
   while (running > 0 && cycles++ <= max_iter + 1) {
-    launch_scale(st, nv, vi, sdd, gs, 1, vec, LH_SLOT_R, 0);
-    for (int j = 0; j < LH_MMAX; ++j) {
-      arnoldi_step(j, mixed);
-      LH_HIP(hipGetLastError());
-      if (read_counter() == 0) break;
-    }
-    launch_backsub(st, S, gs);
-    launch_xupdate(st, nv, vi, sdd, gs, vec);
-    matvec(1, LH_SLOT_X, LH_SLOT_R, 1, false);
-    launch_zero(st, d_counter_.as<int>());
-    launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, d_counter_.as<int>());
+    if (graph) LH_HIP(hipGraphLaunch(cycle_graph_[gi], st));
+    else enqueue_cycle(mixed, true);
     LH_HIP(hipGetLastError());
     running = read_counter();
   }
   LH_HIP(hipStreamSynchronize(st));
+}
+
+// One restart cycle for every running solve (iterative.rs:69-158): v_0 =
+// r / beta, up to 30 Arnoldi steps, back substitution, x += V y, the explicit
+// residual r = rhs - A x (FP64 matvec) and its check, which also opens the
+// next cycle. With early_exit the host stops issuing Arnoldi steps once no
+// solve is left in the loop (one counter read per step); a captured cycle
+// issues all 30.
+void Engine::enqueue_cycle(bool mixed, bool early_exit) {
+  hipStream_t st = stream_;
+  const int nv = (int)vit_.size(), S = (int)sd_.size();
+  const VecItem* vi = d_vit_.as<VecItem>();
+  const SolveDev* sdd = d_solves_.as<SolveDev>();
+  GState* gs = d_state_.as<GState>();
+  double2* vec = d_vec_.as<double2>();
+  launch_scale(st, nv, vi, sdd, gs, 1, vec, LH_SLOT_R, 0);
+  for (int j = 0; j < LH_MMAX; ++j) {
+    arnoldi_step(j, mixed);
+    if (early_exit) {
+      LH_HIP(hipGetLastError());
+      if (read_counter() == 0) break;
+    }
+  }
+  launch_backsub(st, S, gs);
+  launch_xupdate(st, nv, vi, sdd, gs, vec);
+  matvec(1, LH_SLOT_X, LH_SLOT_R, 1, false);
+  launch_zero(st, d_counter_.as<int>());
+  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, d_counter_.as<int>());
 }
 
 std::vector<SolveResult> Engine::results() {

@@ -10,9 +10,9 @@
 //    y += A x + B d (d . x) form -- no tensor is materialised. One target
 //    dipole per lane; source records stream through the scalar cache (every
 //    lane of a wave reads the same record), so a source costs no LDS traffic.
-//    Work items are 256 x 256 tiles of (targets, sources); each writes the
-//    row sums of its targets to the partial slice of its source tile (no
-//    atomics, reproducible results). With a substrate (k_far_row<R, true>,
+//    Work items are tiles of (64, 128 or 256 targets) x (256 sources); each
+//    writes the row sums of its targets to the partial slice of its source
+//    tile (no atomics, reproducible results). With a substrate (k_far_row<R, true>,
 //    assembly.rs:272-295) the image tensor at the mirrored source is applied
 //    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
 //    |d|^2 < 1e-20.
@@ -435,31 +435,31 @@ void launch_prep32(hipStream_t st, int nitems, const VecItem* items, const Solve
 
 void launch_far64(hipStream_t st, bool sub, int nitems, const FarItem* items,
                   const SolveDev* solves, const GState* gs, int which, const Src64* src,
-                  double2* part) {
+                  double2* part, int block) {
   if (nitems <= 0) return;
   if (sub)
-    hipLaunchKernelGGL((k_far_row<double, true, false>), dim3(nitems), dim3(256), 0, st, items, solves,
-                       gs, which, src, part);
+    hipLaunchKernelGGL((k_far_row<double, true, false>), dim3(nitems), dim3(block), 0, st, items,
+                       solves, gs, which, src, part);
   else
-    hipLaunchKernelGGL((k_far_row<double, false, false>), dim3(nitems), dim3(256), 0, st, items, solves,
-                       gs, which, src, part);
+    hipLaunchKernelGGL((k_far_row<double, false, false>), dim3(nitems), dim3(block), 0, st, items,
+                       solves, gs, which, src, part);
 }
 
 void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
                   const SolveDev* solves, const GState* gs, int which, const Src32* src,
-                  double2* part, bool hw_sincos) {
+                  double2* part, bool hw_sincos, int block) {
   if (nitems <= 0) return;
   if (sub && hw_sincos)
-    hipLaunchKernelGGL((k_far_row<float, true, true>), dim3(nitems), dim3(256), 0, st, items,
+    hipLaunchKernelGGL((k_far_row<float, true, true>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
   else if (sub)
-    hipLaunchKernelGGL((k_far_row<float, true, false>), dim3(nitems), dim3(256), 0, st, items,
+    hipLaunchKernelGGL((k_far_row<float, true, false>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
   else if (hw_sincos)
-    hipLaunchKernelGGL((k_far_row<float, false, true>), dim3(nitems), dim3(256), 0, st, items,
+    hipLaunchKernelGGL((k_far_row<float, false, true>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
   else
-    hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(256), 0, st, items,
+    hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
 }
 

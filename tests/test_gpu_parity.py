@@ -101,13 +101,29 @@ def test_matvec_small_and_ragged(backend, n):
     assert rel(s.matvec(x, FP64), O.Problem(pos, alpha, k, True, 2.0).matvec(x)) < 1e-12
 
 
-def test_matvec_fp32_far_field(backend):
-    pos, alpha, k, cell = sphere_problem()
-    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
-    x = rand_x(len(pos), 6)
-    yo = O.Problem(pos, alpha, k, True, cell).matvec(x)
-    e = rel(s.matvec(x, MIXED), yo)
-    print("fp32 far-field matvec rel err", e)
+@pytest.mark.parametrize("sincos", ["hw", "sw"])
+def test_matvec_fp32_far_field(backend, sincos, monkeypatch):
+    """FP32 far field with the hardware sin/cos (default) and with the
+    software sincos_f32."""
+    monkeypatch.setenv("LUMINA_FP32_SINCOS", sincos)
+    for lam in (400.0, 520.0, 800.0):
+        pos, alpha, k, cell = sphere_problem(lam=lam)
+        s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+        x = rand_x(len(pos), 6)
+        yo = O.Problem(pos, alpha, k, True, cell).matvec(x)
+        e = rel(s.matvec(x, MIXED), yo)
+        print(f"fp32 far-field matvec rel err ({sincos}, {lam} nm)", e)
+        assert e < 2e-6
+    # a large object: phases up to k * 95 nm
+    pos = I.au_cuboid_80x40x40_d1p5()
+    alpha = I.polarisability_block("Au", 400.0, 1.5, 1.33)
+    k = I.wavenumber(400.0, 1.33)
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.5)
+    x = rand_x(len(pos), 9)
+    y32 = s.matvec(x, MIXED)
+    y64 = s.matvec(x, FP64)
+    e = rel(y32, y64)
+    print(f"fp32 vs fp64 matvec rel err ({sincos}, cuboid 400 nm)", e)
     assert e < 2e-6
 
 
@@ -216,6 +232,47 @@ def test_batch_resident_rerun(backend):
     assert np.max(np.abs(cs32[:, 0] - cs64[:, 0]) / np.abs(cs64[:, 0])) < 1e-6
     assert kt["far32_launches"] > 0 and kt["far32_ms"] > 0
     assert kt["far32_pairs"] == sum(i["matvecs_fp32"] * 515 * 514 for i in i32)
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
+    """Launch options change scheduling only: the captured-graph cycle and the
+    host-issued cycle, and every far-field block size (the autotuner's
+    choices), give bit-identical moments and iteration counts, because each
+    target's sum over a 256-source tile is formed in the same order."""
+    from lumina_amd.solver import Batch
+    probs = _batch_probs()
+    runs = {}
+    for name, env in [("auto", {}), ("nograph", {"LUMINA_GRAPH": "0"}),
+                      ("b64", {"LUMINA_FAR_BLOCK": "64"}), ("b128", {"LUMINA_FAR_BLOCK": "128"}),
+                      ("b256", {"LUMINA_FAR_BLOCK": "256"})]:
+        for key in ("LUMINA_GRAPH", "LUMINA_FAR_BLOCK"):
+            monkeypatch.delenv(key, raising=False)
+        for key, val in env.items():
+            monkeypatch.setenv(key, val)
+        b = Batch(backend, probs)
+        b.run(1e-10, 200, precision)
+        try:
+            cs, infos, p = b.results(want_moments=True)
+        except ConvergenceFailure as e:
+            cs, infos, p = e.cross_sections, e.infos, e.moments
+        kt = b.kernel_times()
+        b.close()
+        if name.startswith("b"):
+            blk = int(name[1:])
+            assert kt["far64_block"] == blk
+            if precision == MIXED:
+                assert kt["far32_block"] == blk
+        else:
+            assert kt["far64_block"] in (64, 128, 256)
+        runs[name] = (cs, [i["iterations"] for i in infos], p)
+        print(name, runs[name][1], kt["far64_block"], kt["far32_block"], kt["tune_ms"])
+    cs0, it0, p0 = runs["auto"]
+    for name, (cs, it, p) in runs.items():
+        assert it == it0, name
+        assert np.array_equal(cs, cs0), name
+        for a, b_ in zip(p, p0):
+            assert np.array_equal(a, b_), name
 
 
 def test_stored_matrix_session(backend):
