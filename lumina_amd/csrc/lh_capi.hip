@@ -81,8 +81,20 @@ void check_spec(const lh_pairsum_spec* s) {
     throw lh::Error(LH_INVALID_ARGUMENT, "k must be positive and finite");
   if (s->use_fcd && !(s->cell_size > 0.0))
     throw lh::Error(LH_INVALID_ARGUMENT, "cell_size must be positive with FCD");
-  if (s->has_lattice)
-    throw lh::Error(LH_UNAVAILABLE, "periodic (Ewald) pair sums are not built in this library");
+  if (s->has_lattice) {
+    const double cz = s->a1[0] * s->a2[1] - s->a1[1] * s->a2[0];
+    for (int c = 0; c < 3; ++c)
+      if (!std::isfinite(s->a1[c]) || !std::isfinite(s->a2[c]) || !std::isfinite(s->k_bloch[c]))
+        throw lh::Error(LH_INVALID_ARGUMENT, "non-finite lattice or Bloch vector");
+    // a 2D lattice in the xy plane: its spectral sum uses the in-plane
+    // reciprocal vectors (lattice.rs:113-147)
+    if (s->a1[2] != 0.0 || s->a2[2] != 0.0 || !(std::fabs(cz) > 0.0))
+      throw lh::Error(LH_INVALID_ARGUMENT, "lattice vectors must span the xy plane");
+    if (s->truncation_real < 0 || s->truncation_recip < 0 || s->truncation_real > 50 ||
+        s->truncation_recip > 50)
+      throw lh::Error(LH_INVALID_ARGUMENT, "lattice truncations must lie in [0, 50]");
+    if (!std::isfinite(s->ewald_eta)) throw lh::Error(LH_INVALID_ARGUMENT, "non-finite eta");
+  }
   if (s->precision != LH_PREC_FP64 && s->precision != LH_PREC_MIXED)
     throw lh::Error(LH_INVALID_ARGUMENT, "unknown precision");
 }
@@ -97,7 +109,9 @@ lh::Geometry make_geometry(const lh_pairsum_spec* s) {
   // rounding of the positions moves a displacement by far less than 5% of
   // 2 cell_size, which the coordinate bound below guarantees.
   const double R = 2.0 * s->cell_size * 1.05;
-  g.rskip2f = g.use_fcd ? (float)(R * R) : 0.0f;
+  // with a lattice every pair is an Ewald pair: no FCD, no near list
+  // (assembly.rs:244-251)
+  g.rskip2f = (g.use_fcd && !s->has_lattice) ? (float)(R * R) : 0.0f;
   g.x.resize(g.n);
   g.y.resize(g.n);
   g.z.resize(g.n);
@@ -110,10 +124,11 @@ lh::Geometry make_geometry(const lh_pairsum_spec* s) {
       throw lh::Error(LH_INVALID_ARGUMENT, "non-finite dipole position");
     amax = std::max({amax, std::fabs(g.x[i]), std::fabs(g.y[i]), std::fabs(g.z[i])});
   }
-  if (g.use_fcd && amax > 1e4 * s->cell_size)
+  if (g.use_fcd && !s->has_lattice && amax > 1e4 * s->cell_size)
     throw lh::Error(LH_INVALID_ARGUMENT,
                     "positions span more than 1e4 cell sizes from the origin; centre the object");
-  lh::build_near_list(g);
+  if (s->has_lattice) lh::detect_grid(g);
+  else lh::build_near_list(g);
   return g;
 }
 
@@ -126,6 +141,17 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
   p.has_sub = s->has_substrate ? 1 : 0;
   p.z_interface = s->z_interface;
   p.delta = s->has_substrate ? s->delta_eps : lh_c64{0.0, 0.0};
+  if (s->has_lattice) {
+    p.has_lat = 1;
+    for (int c = 0; c < 3; ++c) {
+      p.a1[c] = s->a1[c];
+      p.a2[c] = s->a2[c];
+      p.kb[c] = s->k_bloch[c];
+    }
+    p.eta = s->ewald_eta;
+    p.n_real = s->truncation_real;
+    p.n_recip = s->truncation_recip;
+  }
   return p;
 }
 
@@ -133,13 +159,17 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
 // FP32 far field (default: the hardware instructions where their range
 // allows); LUMINA_GRAPH=0 issues every cycle's kernels from the host instead
 // of replaying a captured graph; LUMINA_FAR_BLOCK=64|128|256 fixes the
-// far-field targets per work item (default: timed on the batch).
+// far-field targets per work item (default: timed on the batch);
+// LUMINA_EWALD=pairs evaluates the Ewald tensor per pair even where the
+// dipoles sit on a grid (default: grid tables).
 lh::Tiling tiling_from_env() {
   lh::Tiling t;
   const char* sc = std::getenv("LUMINA_FP32_SINCOS");
   if (sc && std::string(sc) == "sw") t.hw_sincos = 0;
   const char* gr = std::getenv("LUMINA_GRAPH");
   if (gr && std::string(gr) == "0") t.graph = 0;
+  const char* ew = std::getenv("LUMINA_EWALD");
+  if (ew && std::string(ew) == "pairs") t.ewald_pairs = 1;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
   if (bl) {
     const int b = std::atoi(bl);
@@ -442,6 +472,7 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
       size_t n;
       int fcd;
       double cell;
+      int lat;
     };
     std::vector<Key> keys;
     std::vector<lh::Geometry> geoms;
@@ -457,13 +488,15 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
       int gi = -1;
       for (size_t t = 0; t < keys.size(); ++t)
         if (keys[t].p == sp.positions && keys[t].n == sp.n_dipoles &&
-            keys[t].fcd == (sp.use_fcd ? 1 : 0) && keys[t].cell == sp.cell_size) {
+            keys[t].fcd == (sp.use_fcd ? 1 : 0) && keys[t].cell == sp.cell_size &&
+            keys[t].lat == (sp.has_lattice ? 1 : 0)) {
           gi = (int)t;
           break;
         }
       if (gi < 0) {
         gi = (int)geoms.size();
-        keys.push_back(Key{sp.positions, sp.n_dipoles, sp.use_fcd ? 1 : 0, sp.cell_size});
+        keys.push_back(
+            Key{sp.positions, sp.n_dipoles, sp.use_fcd ? 1 : 0, sp.cell_size, sp.has_lattice ? 1 : 0});
         geoms.push_back(make_geometry(&sp));
       }
       probs.push_back(make_problem(&sp, gi));

@@ -7,6 +7,17 @@
 
 namespace lh {
 
+// Whether solve s takes part in a launch: which = 0 solves inside the
+// Arnoldi loop, 1 solves that start a cycle, 2 solves normalising a new
+// basis vector, 3 every solve.
+__device__ __forceinline__ bool takes_part(const GState* st, int s, int which) {
+  if (which == 3) return true;
+  const GState& g = st[s];
+  if (which == 0) return g.inner != 0;
+  if (which == 1) return g.started != 0;
+  return g.nflag != 0;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

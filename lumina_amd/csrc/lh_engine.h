@@ -42,7 +42,18 @@ struct Geometry {
   std::vector<double> disp;    // [ntab][4] distinct r_i - r_j and the FCD flag
   int ntab = 0;
   long long nnz = 0;
+  // regular grid (lattice solves): x = x0 + ix hx etc. for every dipole
+  int grid = 0;                // 1 when detect_grid found one
+  int gn[3] = {1, 1, 1};       // grid points per axis
+  double g0[3] = {0, 0, 0}, gh[3] = {1, 1, 1};
+  std::vector<int> gpack;      // ix * NYd * NZd + iy * NZd + iz
+  std::vector<int> gizj2;      // 2 iz - (nz - 1)
 };
+
+// Finds a regular grid carrying every dipole (each coordinate within 1e-9
+// relative of x0 + integer * h) and fills the grid fields; leaves grid = 0
+// otherwise.
+void detect_grid(Geometry& g);
 
 // Builds the near list: every pair j != i whose FP32-rounded positions
 // satisfy r2f_exact <= rskip2f, i.e. exactly the pairs the far-field kernels
@@ -59,7 +70,20 @@ struct Problem {
   int has_sub = 0;
   double z_interface = 0.0;
   lh_c64 delta{0.0, 0.0};
+  // 2D lattice (LatticeSpec, lattice.rs:29-44) and Bloch vector
+  int has_lat = 0;
+  double a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+  double eta = 0.0;            // <= 0: sqrt(pi / A)
+  int n_real = 5, n_recip = 5;
+  double kb[3] = {0, 0, 0};
 };
+
+// Ewald records of a lattice problem at its k: the real-space images
+// (EwReal, m1 outer, m2 inner over [-n_real, n_real]) followed by the
+// spectral terms (EwRecip, n1 outer, n2 inner, |q| < 1e-10 left out), as
+// doubles; returns the counts and the splitting parameter eta.
+void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,
+                   double& eta);
 
 // Far-field launch options. block[] is the autotuner's search space.
 struct Tiling {
@@ -67,6 +91,7 @@ struct Tiling {
   int graph = 1;              // replay the restart cycle as a captured HIP graph
   int block[2] = {0, 0};      // far-field targets per work item [FP64, FP32]:
                               // 64, 128 or 256; 0 = timed on the batch at first use
+  int ewald_pairs = 0;        // lattice solves: per-pair evaluation even on a grid
 };
 
 struct SolveResult {
@@ -83,7 +108,7 @@ class Engine {
   void build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t);
   // Re-targets problem s (same geometry): k, alpha^-1, substrate factor.
   void set_problem(int s, const Problem& p);
-  void compute_near_tensors();
+  void compute_near_tensors();   // and the Ewald tables of lattice solves
   int num_solves() const { return (int)probs_.size(); }
   int n_of(int s) const { return geoms_[probs_[s].geom].n; }
 
@@ -145,6 +170,13 @@ class Engine {
       d_src64_, d_src32_, d_part_, d_dot_, d_norm_, d_cross_, d_solves_, d_state_, d_counter_,
       d_vit_, d_tis_, d_tiu_;
   DevMem d_far_free_[2], d_far_sub_[2];
+  // lattice solves: grid-table and per-pair far items, Ewald records, tables
+  std::vector<FarItem> far_ewt_[2], far_ewp_[2];
+  DevMem d_far_ewt_[2], d_far_ewp_[2];
+  DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
+  int n_ewitems_ = 0;
+  std::vector<int> ewl_cap_;   // records reserved per solve
+  std::vector<double> ewl_stage_;   // host staging of a set_problem record update
 };
 
 }  // namespace lh

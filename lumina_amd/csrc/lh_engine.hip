@@ -43,6 +43,15 @@ void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const doub
 void launch_backsub(hipStream_t, int, GState*);
 void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*);
 
+void launch_ewald_table(hipStream_t, int, const SolveDev*, const int*, const long long*,
+                        const double*, double*, float*);
+void launch_ewt64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                  const Src64*, const int*, const int*, const double*, double2*, int);
+void launch_ewt32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                  const Src32*, const int*, const int*, const float*, double2*, int);
+void launch_ewp(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                const Src64*, const double*, double2*, int);
+
 void check_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
   const int st = (e == hipErrorOutOfMemory) ? LH_OUT_OF_MEMORY : LH_DEVICE_ERROR;
@@ -191,6 +200,117 @@ void build_near_list(Geometry& g) {
 // Batch construction.
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Lattice solves: grid detection and Ewald records.
+// ---------------------------------------------------------------------------
+
+void detect_grid(Geometry& g) {
+  g.grid = 0;
+  g.gpack.clear();
+  g.gizj2.clear();
+  if (g.n == 0) return;
+  const std::vector<double>* ax[3] = {&g.x, &g.y, &g.z};
+  std::vector<long long> idx[3];
+  int gn[3];
+  double g0[3], gh[3];
+  for (int a = 0; a < 3; ++a) {
+    const std::vector<double>& v = *ax[a];
+    const double lo = *std::min_element(v.begin(), v.end());
+    const double hi = *std::max_element(v.begin(), v.end());
+    // each coordinate must sit within 1e-12 of the span scale of its grid
+    // line, so that a grid offset times h is the pair's displacement to
+    // about 1e-12 relative
+    const double tol = 1e-12 * std::max({1.0, std::fabs(lo), std::fabs(hi)});
+    std::vector<double> u(v);
+    std::sort(u.begin(), u.end());
+    double h = 0.0;
+    for (size_t q = 1; q < u.size(); ++q) {
+      const double d = u[q] - u[q - 1];
+      if (d > tol && (h == 0.0 || d < h)) h = d;
+    }
+    idx[a].assign(g.n, 0);
+    if (h == 0.0) {
+      gn[a] = 1;
+      g0[a] = lo;
+      gh[a] = 1.0;
+      continue;
+    }
+    const long long m = std::llround((hi - lo) / h);
+    if (m <= 0 || m > (1 << 20)) return;
+    h = (hi - lo) / (double)m;
+    for (int i = 0; i < g.n; ++i) {
+      const long long it = std::llround((v[i] - lo) / h);
+      if (std::fabs(v[i] - (lo + (double)it * h)) > tol) return;
+      idx[a][i] = it;
+    }
+    gn[a] = (int)m + 1;
+    g0[a] = lo;
+    gh[a] = h;
+  }
+  const long long nxd = 2LL * gn[0] - 1, nyd = 2LL * gn[1] - 1, nzd = 2LL * gn[2] - 1;
+  if (nxd * nyd * nzd > (1LL << 30)) return;
+  g.gpack.resize(g.n);
+  g.gizj2.resize(g.n);
+  for (int i = 0; i < g.n; ++i) {
+    g.gpack[i] = (int)(idx[0][i] * nyd * nzd + idx[1][i] * nzd + idx[2][i]);
+    g.gizj2[i] = (int)(2 * idx[2][i] - (gn[2] - 1));
+  }
+  for (int a = 0; a < 3; ++a) {
+    g.gn[a] = gn[a];
+    g.g0[a] = g0[a];
+    g.gh[a] = gh[a];
+  }
+  g.grid = 1;
+}
+
+// lattice.rs:75-147 (cell_measure, eta, reciprocal_vectors), lattice.rs:168-233
+// (point order) and the r-independent factors of ewald.rs:119-273.
+void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,
+                   double& eta) {
+  out.clear();
+  const double cx = p.a1[1] * p.a2[2] - p.a1[2] * p.a2[1];
+  const double cy = p.a1[2] * p.a2[0] - p.a1[0] * p.a2[2];
+  const double cz = p.a1[0] * p.a2[1] - p.a1[1] * p.a2[0];
+  const double area = std::sqrt(cx * cx + cy * cy + cz * cz);
+  eta = (p.eta > 0.0) ? p.eta : std::sqrt(M_PI / area);
+  nreal = 0;
+  for (int m1 = -p.n_real; m1 <= p.n_real; ++m1)
+    for (int m2 = -p.n_real; m2 <= p.n_real; ++m2) {
+      double R[3];
+      for (int c = 0; c < 3; ++c) R[c] = (double)m1 * p.a1[c] + (double)m2 * p.a2[c];
+      const bool zero = std::fabs(R[0]) < 1e-12 && std::fabs(R[1]) < 1e-12 && std::fabs(R[2]) < 1e-12;
+      const double barg = p.kb[0] * R[0] + p.kb[1] * R[1] + p.kb[2] * R[2];
+      out.insert(out.end(), {R[0], R[1], R[2], zero ? 1.0 : 0.0, std::cos(barg), std::sin(barg)});
+      ++nreal;
+    }
+  // reciprocal vectors of the in-plane lattice (lattice.rs:113-147)
+  const double tp = 2.0 * M_PI;
+  const double a2xz[2] = {p.a2[1], -p.a2[0]};
+  const double d1 = p.a1[0] * a2xz[0] + p.a1[1] * a2xz[1];
+  const double b1[2] = {tp * a2xz[0] / d1, tp * a2xz[1] / d1};
+  const double zxa1[2] = {-p.a1[1], p.a1[0]};
+  const double d2 = p.a2[0] * zxa1[0] + p.a2[1] * zxa1[1];
+  const double b2[2] = {tp * zxa1[0] / d2, tp * zxa1[1] / d2};
+  const double pre = 1.0 / (2.0 * area);   // i / (2A) = (0, pre)
+  const double k2 = p.k * p.k;
+  nrecip = 0;
+  for (int n1 = -p.n_recip; n1 <= p.n_recip; ++n1)
+    for (int n2 = -p.n_recip; n2 <= p.n_recip; ++n2) {
+      const double qx = p.kb[0] + ((double)n1 * b1[0] + (double)n2 * b2[0]);
+      const double qy = p.kb[1] + ((double)n1 * b1[1] + (double)n2 * b2[1]);
+      const double x = (qx * qx + qy * qy) - k2;
+      // csqrt of a real argument with Im >= 0 (ewald.rs:334-338)
+      const double qr = x >= 0.0 ? std::sqrt(x) : 0.0;
+      const double qi = x >= 0.0 ? 0.0 : std::sqrt(-x);
+      if (std::sqrt(qr * qr + qi * qi) < 1e-10) continue;
+      // (0, pre) / (qr, qi)
+      const double den = qr * qr + qi * qi;
+      const double cr = pre * qi / den, ci = pre * qr / den;
+      out.insert(out.end(), {qx, qy, qr, qi, cr, ci});
+      ++nrecip;
+    }
+}
+
 Engine::~Engine() {
   clear_events();
   for (auto& g : cycle_graph_)
@@ -225,7 +345,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
 
   std::vector<double> px, py, pz, disp;
-  std::vector<int> ecol, etid;
+  std::vector<int> ecol, etid, gpack, gizj2;
   std::vector<int> g_pos(geoms_.size());
   std::vector<long long> g_ell(geoms_.size()), g_disp(geoms_.size());
   for (size_t gi = 0; gi < geoms_.size(); ++gi) {
@@ -239,6 +359,13 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     ecol.insert(ecol.end(), g.ell_col.begin(), g.ell_col.end());
     etid.insert(etid.end(), g.ell_tid.begin(), g.ell_tid.end());
     disp.insert(disp.end(), g.disp.begin(), g.disp.end());
+    if (g.grid) {
+      gpack.insert(gpack.end(), g.gpack.begin(), g.gpack.end());
+      gizj2.insert(gizj2.end(), g.gizj2.begin(), g.gizj2.end());
+    } else {
+      gpack.insert(gpack.end(), g.n, 0);
+      gizj2.insert(gizj2.end(), g.n, 0);
+    }
   }
 
   sd_.assign(S, SolveDev{});
@@ -246,7 +373,13 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   long long vec_off = 0, src_off = 0, part_off = 0, tab_off = 0, ainv_off = 0;
   int vi_off = 0;
   std::vector<lh_c64> ainv_all;
-  std::vector<int> tis, tiu;
+  std::vector<int> tis, tiu, ewis;
+  std::vector<long long> ewie;
+  std::vector<double> ewl;
+  long long ew_off = 0;
+  ewl_cap_.assign(S, 0);
+  // grid tables beyond these sizes fall back to per-pair evaluation
+  const long long kMaxEntries = 1LL << 24, kMaxTotalEntries = 1LL << 28;
   for (int s = 0; s < S; ++s) {
     const Problem& p = probs_[s];
     const Geometry& g = geoms_[p.geom];
@@ -283,6 +416,42 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
       tis.push_back(s);
       tiu.push_back(u);
     }
+    if (p.has_lat) {
+      std::vector<double> rec;
+      int nr = 0, nq = 0;
+      double eta = 0.0;
+      ewald_records(p, rec, nr, nq, eta);
+      const int cap = nr + (2 * p.n_recip + 1) * (2 * p.n_recip + 1);
+      d.ew_nreal = nr;
+      d.ew_nrecip = nq;
+      d.ew_eta = eta;
+      d.ewl_off = (long long)ewl.size() / 6;
+      ewl_cap_[s] = cap;
+      rec.resize((size_t)cap * 6, 0.0);
+      ewl.insert(ewl.end(), rec.begin(), rec.end());
+      const long long nyd = 2LL * g.gn[1] - 1, nzd = 2LL * g.gn[2] - 1;
+      const long long ewn = (2LL * g.gn[0] - 1) * nyd * nzd;
+      const long long need = ewn * (p.has_sub ? 2 : 1);
+      if (g.grid && !tiling_.ewald_pairs && ewn <= kMaxEntries &&
+          ew_off + need <= kMaxTotalEntries) {
+        d.lat = 1;
+        d.ew_off = ew_off;
+        d.ewn = ewn;
+        d.ew_nyd = (int)nyd;
+        d.ew_nzd = (int)nzd;
+        d.ew_nz1 = g.gn[2] - 1;
+        d.ew_center = (int)((g.gn[0] - 1) * nyd * nzd + (g.gn[1] - 1) * nzd + (g.gn[2] - 1));
+        for (int c = 0; c < 3; ++c) d.ew_h[c] = g.gh[c];
+        d.ew_zimg0 = 2.0 * g.g0[2] - 2.0 * p.z_interface;
+        for (long long e = 0; e < need; e += 256) {
+          ewis.push_back(s);
+          ewie.push_back(e);
+        }
+        ew_off += need;
+      } else {
+        d.lat = 2;
+      }
+    }
     ainv_all.insert(ainv_all.end(), p.ainv.begin(), p.ainv.end());
     ainv_off += (long long)p.ainv.size();
     tab_off += g.ntab;
@@ -292,6 +461,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     vi_off += d.nvi;
   }
   n_titems_ = (int)tis.size();
+  n_ewitems_ = (int)ewis.size();
 
   hipStream_t st = stream_;
   upload(d_px_, px, st);
@@ -305,6 +475,13 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   upload(d_vit_, vit_, st);
   upload(d_tis_, tis, st);
   upload(d_tiu_, tiu, st);
+  upload(d_gpack_, gpack, st);
+  upload(d_gizj2_, gizj2, st);
+  upload(d_ewl_, ewl, st);
+  upload(d_ewi_s_, ewis, st);
+  upload(d_ewi_e_, ewie, st);
+  d_ewt64_.alloc(sizeof(double) * 12 * (size_t)std::max<long long>(ew_off, 1));
+  d_ewt32_.alloc(sizeof(float) * 12 * (size_t)std::max<long long>(ew_off, 1));
   d_tab_.alloc(sizeof(double) * 12 * (size_t)std::max<long long>(tab_off, 1));
   d_vec_.alloc(sizeof(double2) * (size_t)std::max<long long>(vec_off, 1));
   LH_HIP(hipMemsetAsync(d_vec_.p, 0, d_vec_.bytes, st));
@@ -325,6 +502,8 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
 void Engine::build_far_items(int p) {
   far_free_[p].clear();
   far_sub_[p].clear();
+  far_ewt_[p].clear();
+  far_ewp_[p].clear();
   // before tuning, any block size serves (the far items are rebuilt then)
   const int tb = tiling_.block[p] > 0 ? tiling_.block[p] : LH_TILE;
   for (size_t s = 0; s < sd_.size(); ++s) {
@@ -332,13 +511,17 @@ void Engine::build_far_items(int p) {
     for (int ti = 0; ti < (d.n + tb - 1) / tb; ++ti)
       for (int tj = 0; tj < d.nt; ++tj) {
         FarItem f{(int)s, ti * tb, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
-        if (d.has_sub) far_sub_[p].push_back(f);
+        if (d.lat == 1) far_ewt_[p].push_back(f);
+        else if (d.lat == 2) far_ewp_[p].push_back(f);
+        else if (d.has_sub) far_sub_[p].push_back(f);
         else far_free_[p].push_back(f);
       }
   }
   drop_graphs();
   upload(d_far_free_[p], far_free_[p], stream_);
   upload(d_far_sub_[p], far_sub_[p], stream_);
+  upload(d_far_ewt_[p], far_ewt_[p], stream_);
+  upload(d_far_ewp_[p], far_ewp_[p], stream_);
 }
 
 void Engine::drop_graphs() {
@@ -406,6 +589,19 @@ void Engine::set_problem(int s, const Problem& p) {
   d.c0 = p.k * p.k / (4.0 * M_PI);
   d.dre = p.delta.re;
   d.dim = p.delta.im;
+  if (q.has_lat) {
+    std::vector<double> rec;
+    int nr = 0, nq = 0;
+    double eta = 0.0;
+    ewald_records(q, rec, nr, nq, eta);
+    if (nr + nq > ewl_cap_[s]) throw Error(LH_DEVICE_ERROR, "Ewald record overflow");
+    d.ew_nreal = nr;
+    d.ew_nrecip = nq;
+    d.ew_eta = eta;
+    ewl_stage_ = rec;
+    LH_HIP(hipMemcpyAsync(d_ewl_.as<double>() + 6 * d.ewl_off, ewl_stage_.data(),
+                          ewl_stage_.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+  }
   LH_HIP(hipMemcpyAsync(d_ainv_.as<double2>() + d.ainv_off, q.ainv.data(),
                         q.ainv.size() * sizeof(lh_c64), hipMemcpyHostToDevice, stream_));
   LH_HIP(hipMemcpyAsync(d_solves_.as<SolveDev>() + s, &d, sizeof(SolveDev),
@@ -417,6 +613,9 @@ void Engine::set_problem(int s, const Problem& p) {
 void Engine::compute_near_tensors() {
   launch_near_tensors(stream_, n_titems_, d_solves_.as<SolveDev>(), d_tis_.as<int>(),
                       d_tiu_.as<int>(), d_disp_.as<double>(), d_tab_.as<double>());
+  launch_ewald_table(stream_, n_ewitems_, d_solves_.as<SolveDev>(), d_ewi_s_.as<int>(),
+                     d_ewi_e_.as<long long>(), d_ewl_.as<double>(), d_ewt64_.as<double>(),
+                     d_ewt32_.as<float>());
   LH_HIP(hipGetLastError());
 }
 
@@ -456,7 +655,9 @@ void Engine::fill_incident(int s, const lh_incident& inc) {
 
 double Engine::pairs_per_matvec(int s) const {
   const double n = geoms_[probs_[s].geom].n;
-  return n * (n - 1.0) + (probs_[s].has_sub ? n * n : 0.0);
+  // with a lattice the self block is a lattice sum too (assembly.rs:241-243)
+  const double direct = probs_[s].has_lat ? n * n : n * (n - 1.0);
+  return direct + (probs_[s].has_sub ? n * n : 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,14 +679,22 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     LH_HIP(hipEventCreate(&ev.a));
     LH_HIP(hipEventCreate(&ev.b));
   }
+  const int newt = (int)far_ewt_[p].size(), newp = (int)far_ewp_[p].size();
   if (fp32) {
     launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
+    // the per-pair lattice form is FP64 in either precision
+    if (newp > 0)
+      launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
+                    d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
     launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
                  d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
     launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
                  d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+    launch_ewt32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
+                 d_gpack_.as<int>(), d_gizj2_.as<int>(), d_ewt32_.as<float>(),
+                 d_part_.as<double2>(), blk);
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
@@ -494,7 +703,12 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    launch_ewt64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
+                 d_gpack_.as<int>(), d_gizj2_.as<int>(), d_ewt64_.as<double>(),
+                 d_part_.as<double2>(), blk);
   }
+  launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
+             d_ewl_.as<double>(), d_part_.as<double2>(), blk);
   if (timing_) {
     LH_HIP(hipEventRecord(ev.b, st));
     evs_.push_back(ev);

@@ -92,13 +92,6 @@ __global__ __launch_bounds__(256) void k_near_tensors(const SolveDev* __restrict
 // w -> v_{j+1}, 3 = always (single matvec).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ bool takes_part(const GState* st, int s, int which) {
-  if (which == 3) return true;
-  const GState& g = st[s];
-  if (which == 0) return g.inner != 0;
-  if (which == 1) return g.started != 0;
-  return g.nflag != 0;
-}
 
 // ---------------------------------------------------------------------------
 // Source records for one matvec: positions + input vector (+ image-scaled
@@ -171,22 +164,23 @@ __device__ __forceinline__ float fy(const Src32& r) { return r.y; }
 __device__ __forceinline__ float fz(const Src32& r) { return r.z; }
 
 // G(d) = A I + B d d^T (greens.rs:34-71), scaled by `scale`.
-template <bool HW = false>
 __device__ __forceinline__ void coeffs(double r2, double scale, double k, double inv_k, double c0,
                                        double& Ar, double& Ai, double& Br, double& Bi) {
   g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi);
   Ar *= scale; Ai *= scale; Br *= scale; Bi *= scale;
 }
-// HW = true: sin/cos from the hardware v_sin_f32/v_cos_f32 (argument in
-// revolutions, kr / 2 pi); otherwise sincos_f32.
+// FP32: skipped pairs get 1/r = 0 (then A = B = 0; r2 is finite, so
+// r = r2 / r stays finite). HW = true: sin/cos from the hardware
+// v_sin_f32/v_cos_f32 with the phase in revolutions (k / 2 pi = krev);
+// otherwise sincos_f32.
 template <bool HW = false>
-__device__ __forceinline__ void coeffs(float r2, float scale, float k, float inv_k, float c0,
-                                       float& Ar, float& Ai, float& Br, float& Bi) {
-  const float inv_r = __builtin_amdgcn_rsqf(r2);
+__device__ __forceinline__ void coeffs32(float r2, bool skip, float k, float krev, float inv_k,
+                                         float c0, float& Ar, float& Ai, float& Br, float& Bi) {
+  const float inv_r = skip ? 0.0f : __builtin_amdgcn_rsqf(r2);
   const float r = r2 * inv_r;
   float s, c;
   if constexpr (HW) {
-    const float rev = (k * r) * 0.15915494309189535f;
+    const float rev = r * krev;
     s = __builtin_amdgcn_sinf(rev);
     c = __builtin_amdgcn_cosf(rev);
   } else {
@@ -194,7 +188,7 @@ __device__ __forceinline__ void coeffs(float r2, float scale, float k, float inv
   }
   const float u = inv_r * inv_k;       // 1 / (k r)
   const float u2 = u * u;
-  const float pf = c0 * inv_r * scale; // k^2 / (4 pi r)
+  const float pf = c0 * inv_r;         // k^2 / (4 pi r)
   const float pr = pf * c, pi = pf * s;
   const float ar = 1.0f - u2, ai = u;
   const float br = 3.0f * u2 - 1.0f, bi = -3.0f * u;
@@ -273,28 +267,40 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
   const float xif = fx(me), yif = fy(me), zif = fz(me);
   const R xi = me.x, yi = me.y, zi = me.z;
   const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0, z2 = (R)S.z2;
+  const float krev = (float)(S.k * 0.15915494309189535);
   const float rskip2f = S.rskip2f;
   RowAcc<R> row;
   row.zero();
+  // the next source record is loaded while the current one is used
+  SRC nxt = sp[it.c0];
   for (int j0 = it.c0; j0 < it.c1; j0 += LH_RUN) {
     const int j1 = min(j0 + LH_RUN, it.c1);
     R acc[6] = {0, 0, 0, 0, 0, 0};
     for (int j = j0; j < j1; ++j) {
-      const SRC& r = sp[j];
+      const SRC r = nxt;
+      nxt = sp[min(j + 1, it.c1 - 1)];
       const float d2f = r2f_exact(xif - fx(r), yif - fy(r), zif - fz(r));
-      const bool skip = (d2f <= rskip2f) || (j == i);
       const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
-      // FP32: the exclusion test's |d|^2 is the value's |d|^2
-      const R r2 = std::is_same<R, float>::value ? (R)d2f : dx * dx + dy * dy + dz * dz;
       R Ar, Ai, Br, Bi;
-      coeffs<HW>(skip ? R(1) : r2, skip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+      if constexpr (std::is_same<R, float>::value) {
+        // d2f <= rskip2f also holds for j == i (d2f = 0, rskip2f >= 0); the
+        // exclusion test's |d|^2 is the value's |d|^2
+        coeffs32<HW>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
+      } else {
+        const bool skip = (d2f <= rskip2f) || (j == i);
+        const R r2 = dx * dx + dy * dy + dz * dz;
+        coeffs(skip ? R(1) : r2, skip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+      }
       apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, r.v, acc);
       if (SUB) {
         // image of source j: (x_j, y_j, 2 z0 - z_j)
         const R ez = zi - (z2 - r.z);
         const R e2 = dx * dx + dy * dy + ez * ez;
         const bool eskip = e2 < R(1e-20);
-        coeffs<HW>(eskip ? R(1) : e2, eskip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+        if constexpr (std::is_same<R, float>::value)
+          coeffs32<HW>(e2, eskip, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
+        else
+          coeffs(eskip ? R(1) : e2, eskip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
         apply<R>(dx, dy, ez, Ar, Ai, Br, Bi, r.w, acc);
       }
     }

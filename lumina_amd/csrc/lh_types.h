@@ -19,6 +19,13 @@
 //   src64 / src32       [sum over solves of n]        per-matvec source records
 //   part                [sum over solves of nt*3n]    far-field partial sums, one slice per
 //                                                     256-dipole tile
+//   gpack / gizj2       [sum over geometries of n]    lattice solves on a regular grid:
+//                                                     packed grid coordinates of each dipole
+//   ewl                 [sum over lattice solves]     Ewald real-space image records and
+//                                                     spectral records (EwReal, EwRecip)
+//   ewt64 / ewt32       [sum over lattice solves of   periodic tensors per grid offset
+//                        (1 + sub) * ewn * 12]        (xx yy zz xy xz yz, re/im): direct
+//                                                     table, then the Fresnel image table
 #pragma once
 #include <stdint.h>
 
@@ -61,6 +68,32 @@ struct SolveDev {
   double dre, dim;     // Fresnel factor Delta
   double cell;         // FCD cell size
   double tol;
+  // 2D periodic lattice (Ewald) solves; lat = 0 for free-space solves
+  int lat;             // 0 none, 1 grid tables, 2 per-pair evaluation
+  int ew_nreal, ew_nrecip;   // real-space images, spectral terms
+  int ew_nz1;          // grid points along z minus one
+  long long ew_off;    // first table entry of this solve (direct; image at + ewn)
+  long long ewn;       // entries per table = NXd * NYd * NZd
+  long long ewl_off;   // first EwReal record (EwRecip records follow)
+  int ew_nyd, ew_nzd;  // table extents along y and z (2 n - 1)
+  int ew_center;       // table index of the zero offset
+  int pad2;
+  double ew_eta;
+  double ew_h[3];      // grid spacing
+  double ew_zimg0;     // 2 z_min - 2 z_interface (image z at iz_i + iz_j = 0)
+};
+
+// Real-space image R of the Ewald sum (ewald.rs:119-174) with its Bloch
+// phase e^{i k_par . R}; zero = 1 for the R = 0 image.
+struct EwReal {
+  double x, y, z, zero, br, bi;
+};
+
+// Spectral term (ewald.rs:186-273) at Q = k_par + G: (Qx, Qy), q =
+// csqrt+(|Q|^2 - k^2) and c = (i / 2A) / q. Terms with |q| < 1e-10 are
+// left out of the list.
+struct EwRecip {
+  double qx, qy, qr, qi, cr, ci;
 };
 
 // Far-field work item: targets [t0, t0 + 256) against sources [c0, c1) of

@@ -84,8 +84,8 @@ def substrate_reflection_factor(eps_sub: complex, eps_env: float, use_retarded: 
 
 @dataclass
 class LatticeSpec:
-    """lattice.rs:29-44 (2D). Accepted by the API for completeness; the
-    periodic pair sum is not built in this library and raises Unavailable."""
+    """lattice.rs:29-44 (2D lattice in the xy plane). ewald_eta None or <= 0
+    selects sqrt(pi / A) (lattice.rs:96-102)."""
     a1: tuple
     a2: tuple
     ewald_eta: float | None = None
@@ -380,7 +380,8 @@ class Batch:
     (lh_batch_create / lh_batch_run / lh_batch_results).
 
     ``problems``: list of dicts with keys positions, polarisability, k,
-    use_fcd, cell_size, incident (IncidentField), substrate (optional).
+    use_fcd, cell_size, incident (IncidentField), substrate (optional),
+    lattice (LatticeSpec, optional) and k_bloch (optional).
     Problems passing the same ``positions`` array object share its geometry.
     """
 
@@ -395,7 +396,8 @@ class Batch:
             if pid not in pos_cache:
                 pos_cache[pid] = _f64(pr["positions"]).reshape(-1, 3)
             sp, kp = _spec(pos_cache[pid], pr["polarisability"], pr["k"], pr.get("use_fcd", True),
-                           pr.get("cell_size", 3.0), pr.get("substrate"), FP64)
+                           pr.get("cell_size", 3.0), pr.get("substrate"), FP64,
+                           pr.get("lattice"), pr.get("k_bloch", (0.0, 0.0, 0.0)))
             keep.append(kp)
             arr[q].spec = sp
             arr[q].incident = pr.get("incident", IncidentField())._c()
