@@ -107,6 +107,11 @@ typedef struct {
   double z_interface;
   lh_c64 delta_eps;               /* reflection factor (substrate.rs:191-197) */
   int precision;                  /* lh_precision */
+  /* Optional [n][3] positions on which the FCD near/far decision of each
+   * pair is taken (NULL: the positions themselves). No reference
+   * counterpart: it holds the classification fixed while a dipole is moved,
+   * for finite differences and shape derivatives. */
+  const double* classify_positions;
 } lh_pairsum_spec;
 
 /* Incident plane wave (IncidentField, types.rs:37-74). The polarisation is
@@ -164,6 +169,23 @@ int lh_pairsum_cross_sections(lh_pairsum* s, const lh_c64* p, const lh_c64* e_in
 int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, double tol,
                                       size_t max_iter, double out[3], lh_c64* p_out,
                                       lh_solve_info* info);
+/* Adjoint gradients of C_ext (no reference counterpart; the backward of the
+ * solve, as a torch.autograd.Function would expose it). Solves A p = E (the
+ * forward solve of lh_pairsum_compute_cross_sections) and A lam = conj(E)
+ * on the session's operator, which is complex symmetric, then
+ *   grad_eps[2i], grad_eps[2i+1] = dC_ext/dRe(eps_i), dC_ext/dIm(eps_i),
+ *     from dainv_deps = d(alpha_i^-1)/d eps_i ([n][9] or [9] if
+ *     dainv_uniform, row-major complex: the caller's material model);
+ *   grad_pos[3i+c] = dC_ext/d r_i,c, with the FCD near/far classification
+ *     of every pair held fixed; NULL skips it.
+ * cross_sections receives C_ext, C_abs, C_sca of the forward solve.
+ * LH_UNAVAILABLE: grad_pos with a substrate or lattice, or a lattice with a
+ * non-zero Bloch vector (operator not symmetric). LH_CONVERGENCE_FAILURE if
+ * either solve hits max_iter (outputs are still written). */
+int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, size_t max_iter,
+                            const lh_c64* dainv_deps, int dainv_uniform,
+                            double cross_sections[3], double* grad_eps, double* grad_pos,
+                            lh_solve_info* forward_info, lh_solve_info* adjoint_info);
 void lh_pairsum_destroy(lh_pairsum* s);
 
 /* ---- batched solves ------------------------------------------------------ */

@@ -1,0 +1,369 @@
+// Adjoint gradients of C_ext (no reference counterpart: the reference has no
+// derivatives).
+//
+// C_ext = (k / E0^2) Im(E^H p) with A p = E (cda/mod.rs:187-199). The
+// operator is complex symmetric in free space, with the Fresnel image and
+// with a Bloch-periodic lattice at k_par = 0, so the adjoint solve is
+// A lam = conj(E) on the same engine, and for any parameter theta
+//   dC/dtheta = (k / E0^2) Im(dE^H p + lam^T dE - lam^T dA p).
+//  * eps_i (through alpha_i^-1, caller-supplied D_i = d alpha_i^-1 / d eps_i):
+//    f = -lam_i^T D_i p_i; dC/dRe eps_i = K Im f, dC/dIm eps_i = K Re f.
+//  * r_i (free space): E_i = E0 e e^{ik dir.r_i} and the pairs (i, j), (j, i)
+//    give
+//      dC/dr_ic = K Im( -i k dir_c conj(E_i).p_i + i k dir_c lam_i.E_i
+//                      + sum_{j != i} [lam_i^T dG_c p_j + p_i^T dG_c lam_j] ),
+//    dG_c = d G(r_i - r_j) / d r_ic (FCD average where the forward used it;
+//    the near/far classification is held fixed).
+// Point-tensor derivative, with G = A I + B d d^T (A, B as in g_coeffs64),
+// u = 1/(kr), e = c0 e^{ikr}:
+//   dG_ab/dr_c = Ap d_c delta_ab + Bp d_c d_a d_b + B (delta_ac d_b + delta_bc d_a),
+//   Ap = A'(r)/r = e r^-3 [(3u^2 - 2) + i (kr - 3u)],
+//   Bp = B'(r)/r = e r^-5 [(6 - 15u^2) + i (15u - kr)],
+//   B  = e r^-3 [(3u^2 - 1) - 3iu].
+#include "lh_device.h"
+
+namespace lh {
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return cplx{a.re * s, a.im * s}; }
+// a . b over 3 components (bilinear, no conjugate), interleaved (re, im)
+__device__ __forceinline__ cplx dot3(const double* a, const double* b) {
+  cplx s{0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s = cadd(s, cmul(cplx{a[2 * c], a[2 * c + 1]}, cplx{b[2 * c], b[2 * c + 1]}));
+  return s;
+}
+// d . v for a real d
+__device__ __forceinline__ cplx ddot(const double d[3], const double* v) {
+  return cplx{d[0] * v[0] + d[1] * v[2] + d[2] * v[4], d[0] * v[1] + d[1] * v[3] + d[2] * v[5]};
+}
+
+// Ap, Bp, B of the point tensor at displacement d (|d|^2 = r2 > 0).
+__device__ __forceinline__ void grad_coeffs(double r2, double k, double inv_k, double c0, cplx& Ap,
+                                            cplx& Bp, cplx& B) {
+  const double inv_r = rsqrt(r2);
+  const double r = r2 * inv_r;
+  const double kr = k * r;
+  double sn, cs;
+  sincos_f64(kr, sn, cs);
+  const double u = inv_r * inv_k;
+  const double u2 = u * u;
+  const double ir3 = inv_r * inv_r * inv_r;
+  const double ir5 = ir3 * inv_r * inv_r;
+  const cplx e{c0 * cs, c0 * sn};
+  Ap = cmul(cscale(e, ir3), cplx{3.0 * u2 - 2.0, kr - 3.0 * u});
+  Bp = cmul(cscale(e, ir5), cplx{6.0 - 15.0 * u2, 15.0 * u - kr});
+  B = cmul(cscale(e, ir3), cplx{3.0 * u2 - 1.0, -3.0 * u});
+}
+
+// dG_c as symmetric tensors: t[c][q] for q in (xx yy zz xy xz yz), 18 complex.
+__device__ __forceinline__ void grad_tensor(const double d[3], cplx Ap, cplx Bp, cplx B,
+                                            cplx t[3][6]) {
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int a = ia[q], b = ib[q];
+      cplx v = cscale(Bp, d[c] * d[a] * d[b]);
+      if (a == b) v = cadd(v, cscale(Ap, d[c]));
+      double w = 0.0;
+      if (a == c) w += d[b];
+      if (b == c) w += d[a];
+      t[c][q] = cadd(v, cscale(B, w));
+    }
+}
+
+// x^T T y for a symmetric complex 3x3 T stored as (xx yy zz xy xz yz).
+__device__ __forceinline__ cplx sym_form(const cplx* t, const double* x, const double* y) {
+  const int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  cplx s{0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      s = cadd(s, cmul(cmul(cplx{x[2 * a], x[2 * a + 1]}, t[idx[a][b]]), cplx{y[2 * b], y[2 * b + 1]}));
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Near-field gradient tensors, one thread per distinct displacement (the
+// same table of displacements as the forward near tensors).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_near_grad_tensors(const SolveDev* __restrict__ solves,
+                                                          const int* __restrict__ item_s,
+                                                          const int* __restrict__ item_u,
+                                                          const double* __restrict__ disp,
+                                                          double* __restrict__ gtab) {
+  const SolveDev S = solves[item_s[blockIdx.x]];
+  const int u = item_u[blockIdx.x] + threadIdx.x;
+  if (u >= S.ntab) return;
+  const double* dd = disp + 4 * (S.dtab_off + u);
+  cplx acc[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[c][q] = cplx{0.0, 0.0};
+  if (dd[3] != 0.0) {
+    // greens.rs:120-164 nodes and weights, normalised by the total weight
+    const double half_d = S.cell / 2.0;
+    const double nodes[3] = {-sqrt(3.0 / 5.0), 0.0, sqrt(3.0 / 5.0)};
+    const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    double tw = 0.0;
+    for (int ix = 0; ix < 3; ++ix)
+      for (int iy = 0; iy < 3; ++iy)
+        for (int iz = 0; iz < 3; ++iz) {
+          const double e[3] = {dd[0] - half_d * nodes[ix], dd[1] - half_d * nodes[iy],
+                               dd[2] - half_d * nodes[iz]};
+          const double r2 = r2_exact(e[0], e[1], e[2]);
+          if (r2 < 1e-20) continue;
+          const double w = wts[ix] * wts[iy] * wts[iz];
+          cplx Ap, Bp, B;
+          grad_coeffs(r2, S.k, S.inv_k, S.c0, Ap, Bp, B);
+          cplx t[3][6];
+          grad_tensor(e, Ap, Bp, B, t);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[c][q] = cadd(acc[c][q], cscale(t[c][q], w));
+          tw += w;
+        }
+    if (tw > 1e-30) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[c][q] = cscale(acc[c][q], 1.0 / tw);
+    }
+  } else {
+    const double e[3] = {dd[0], dd[1], dd[2]};
+    cplx Ap, Bp, B;
+    grad_coeffs(r2_exact(e[0], e[1], e[2]), S.k, S.inv_k, S.c0, Ap, Bp, B);
+    grad_tensor(e, Ap, Bp, B, acc);
+  }
+  double* o = gtab + 36 * (S.tab_off + u);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      o[12 * c + 2 * q] = acc[c][q].re;
+      o[12 * c + 2 * q + 1] = acc[c][q].im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Source records for the gradient pass: v = p_j (slot p_slot), w = lam_j
+// (slot X).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(LH_VB) void k_prep_grad(const VecItem* __restrict__ items,
+                                                    const SolveDev* __restrict__ solves,
+                                                    const double* __restrict__ px,
+                                                    const double* __restrict__ py,
+                                                    const double* __restrict__ pz,
+                                                    const double2* __restrict__ vec, int p_slot,
+                                                    Src64* __restrict__ src) {
+  const VecItem it = items[blockIdx.x];
+  const SolveDev S = solves[it.s];
+  const int j = it.t0 + threadIdx.x;
+  if (j >= S.n) return;
+  const long long n3 = 3LL * S.n;
+  const double2* P = vec + S.vec_off + (long long)p_slot * n3 + 3LL * j;
+  const double2* L = vec + S.vec_off + (long long)LH_SLOT_X * n3 + 3LL * j;
+  Src64 r;
+  const double x = px[S.pos_off + j], y = py[S.pos_off + j], z = pz[S.pos_off + j];
+  r.x = x; r.y = y; r.z = z;
+  r.xf = (float)x; r.yf = (float)y; r.zf = (float)z; r.pf = 0.0f;
+  r.pad = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 a = P[c], b = L[c];
+    r.v[2 * c] = a.x; r.v[2 * c + 1] = a.y;
+    r.w[2 * c] = b.x; r.w[2 * c + 1] = b.y;
+  }
+  src[S.src_off + j] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Far pairs: per target i, g_c = sum_j [lam_i^T dG_c p_j + p_i^T dG_c lam_j]
+// over the pairs outside the exclusion radius (the same test as the forward
+// far field), written to the partial slice of the source tile.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_grad_far(const FarItem* __restrict__ items,
+                                                  const SolveDev* __restrict__ solves,
+                                                  const Src64* __restrict__ src,
+                                                  double2* __restrict__ part) {
+  const FarItem it = items[blockIdx.x];
+  const SolveDev S = solves[it.s];
+  const int i = it.t0 + threadIdx.x;
+  const bool valid = i < S.n;
+  const Src64* __restrict__ sp = src + S.src_off;
+  const Src64& me = sp[valid ? i : 0];
+  double pi_[6], li[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    pi_[q] = me.v[q];
+    li[q] = me.w[q];
+  }
+  cplx g[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int j = it.c0; j < it.c1; ++j) {
+    const Src64& r = sp[j];
+    const float d2f = r2f_exact(me.xf - r.xf, me.yf - r.yf, me.zf - r.zf);
+    if ((d2f <= S.rskip2f) || (j == i)) continue;
+    const double d[3] = {me.x - r.x, me.y - r.y, me.z - r.z};
+    cplx Ap, Bp, B;
+    grad_coeffs(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], S.k, S.inv_k, S.c0, Ap, Bp, B);
+    const cplx dl = ddot(d, li), dp = ddot(d, pi_), dpj = ddot(d, r.v), dlj = ddot(d, r.w);
+    const cplx s1 = cadd(dot3(li, r.v), dot3(pi_, r.w));
+    const cplx s2 = cadd(cmul(dl, dpj), cmul(dp, dlj));
+    const cplx T = cadd(cmul(Ap, s1), cmul(Bp, s2));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx w = cmul(dpj, cplx{li[2 * c], li[2 * c + 1]});
+      w = cadd(w, cmul(dl, cplx{r.v[2 * c], r.v[2 * c + 1]}));
+      w = cadd(w, cmul(dlj, cplx{pi_[2 * c], pi_[2 * c + 1]}));
+      w = cadd(w, cmul(dp, cplx{r.w[2 * c], r.w[2 * c + 1]}));
+      g[c] = cadd(g[c], cadd(cscale(T, d[c]), cmul(B, w)));
+    }
+  }
+  if (valid) {
+    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = make_double2(g[c].re, g[c].im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per dipole: far partials + near pairs + incident-field term, and the
+// permittivity term. out[5 i + 0, 1] = dC/dRe eps_i, dC/dIm eps_i;
+// out[5 i + 2 .. 4] = dC/dr_i (0 when positions == 0).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(LH_VB) void k_grad_reduce(
+    const VecItem* __restrict__ items, const SolveDev* __restrict__ solves,
+    const int* __restrict__ ell_col, const int* __restrict__ ell_tid,
+    const double* __restrict__ gtab, const double2* __restrict__ vec, int p_slot,
+    const double2* __restrict__ part, const double2* __restrict__ dainv, int dainv_uniform,
+    long long dainv_off, double dx, double dy, double dz, double K, int positions,
+    double* __restrict__ out) {
+  const VecItem it = items[blockIdx.x];
+  const SolveDev S = solves[it.s];
+  const int i = it.t0 + threadIdx.x;
+  if (i >= S.n) return;
+  const long long n3 = 3LL * S.n;
+  const double2* P = vec + S.vec_off + (long long)p_slot * n3;
+  const double2* L = vec + S.vec_off + (long long)LH_SLOT_X * n3;
+  const double2* E = vec + S.vec_off + (long long)LH_SLOT_E * n3;
+  double pi_[6], li[6], ei[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 a = P[3LL * i + c], b = L[3LL * i + c], e = E[3LL * i + c];
+    pi_[2 * c] = a.x; pi_[2 * c + 1] = a.y;
+    li[2 * c] = b.x; li[2 * c + 1] = b.y;
+    ei[2 * c] = e.x; ei[2 * c + 1] = e.y;
+  }
+  // permittivity: f = -lam_i^T D_i p_i
+  const double2* D = dainv + dainv_off + (dainv_uniform ? 0 : 9LL * i);
+  cplx f{0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double2 m = D[3 * a + b];
+      f = cadd(f, cmul(cmul(cplx{li[2 * a], li[2 * a + 1]}, cplx{m.x, m.y}),
+                       cplx{pi_[2 * b], pi_[2 * b + 1]}));
+    }
+  double* o = out + 5LL * (S.src_off + i);
+  o[0] = -K * f.im;
+  o[1] = -K * f.re;
+  if (!positions) {
+    o[2] = o[3] = o[4] = 0.0;
+    return;
+  }
+  cplx g[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int ch = 0; ch < S.nt; ++ch) {
+    const double2* q = part + S.part_off + (long long)ch * n3 + 3LL * i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c] = cadd(g[c], cplx{q[c].x, q[c].y});
+  }
+  for (int kk = 0; kk < S.kw; ++kk) {
+    const long long e = S.ell_off + (long long)kk * S.n + i;
+    const int u = ell_tid[e];
+    if (u < 0) break;
+    const int j = ell_col[e];
+    double pj[6], lj[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 a = P[3LL * j + c], b = L[3LL * j + c];
+      pj[2 * c] = a.x; pj[2 * c + 1] = a.y;
+      lj[2 * c] = b.x; lj[2 * c + 1] = b.y;
+    }
+    const double* t = gtab + 36 * (S.tab_off + u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx tc[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) tc[q] = cplx{t[12 * c + 2 * q], t[12 * c + 2 * q + 1]};
+      g[c] = cadd(g[c], cadd(sym_form(tc, li, pj), sym_form(tc, pi_, lj)));
+    }
+  }
+  // incident field: -i k dir_c conj(E_i).p_i + i k dir_c lam_i.E_i
+  cplx t1{0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    t1 = cadd(t1, cmul(cplx{ei[2 * c], -ei[2 * c + 1]}, cplx{pi_[2 * c], pi_[2 * c + 1]}));
+  const cplx t2 = dot3(li, ei);
+  const double dir[3] = {dx, dy, dz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double kd = S.k * dir[c];
+    g[c] = cadd(g[c], cadd(cmul(cplx{0.0, -kd}, t1), cmul(cplx{0.0, kd}, t2)));
+    o[2 + c] = K * g[c].im;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (host).
+// ---------------------------------------------------------------------------
+
+void launch_near_grad_tensors(hipStream_t st, int nitems, const SolveDev* solves,
+                              const int* item_s, const int* item_u, const double* disp,
+                              double* gtab) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_near_grad_tensors, dim3(nitems), dim3(256), 0, st, solves, item_s, item_u,
+                     disp, gtab);
+}
+
+void launch_prep_grad(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
+                      const double* px, const double* py, const double* pz, const double2* vec,
+                      int p_slot, Src64* src) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_prep_grad, dim3(nitems), dim3(LH_VB), 0, st, items, solves, px, py, pz,
+                     vec, p_slot, src);
+}
+
+void launch_grad_far(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                     const Src64* src, double2* part, int block) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_grad_far, dim3(nitems), dim3(block), 0, st, items, solves, src, part);
+}
+
+void launch_grad_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
+                        const int* ell_col, const int* ell_tid, const double* gtab,
+                        const double2* vec, int p_slot, const double2* part,
+                        const double2* dainv, int dainv_uniform, long long dainv_off,
+                        const double dir[3], double K, int positions, double* out) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_grad_reduce, dim3(nitems), dim3(LH_VB), 0, st, items, solves, ell_col,
+                     ell_tid, gtab, vec, p_slot, part, dainv, dainv_uniform, dainv_off, dir[0],
+                     dir[1], dir[2], K, positions, out);
+}
+
+}  // namespace lh

@@ -124,6 +124,11 @@ lh::Geometry make_geometry(const lh_pairsum_spec* s) {
       throw lh::Error(LH_INVALID_ARGUMENT, "non-finite dipole position");
     amax = std::max({amax, std::fabs(g.x[i]), std::fabs(g.y[i]), std::fabs(g.z[i])});
   }
+  if (s->classify_positions) {
+    g.cls.assign(s->classify_positions, s->classify_positions + 3 * (size_t)g.n);
+    for (double v : g.cls)
+      if (!std::isfinite(v)) throw lh::Error(LH_INVALID_ARGUMENT, "non-finite classify position");
+  }
   if (g.use_fcd && !s->has_lattice && amax > 1e4 * s->cell_size)
     throw lh::Error(LH_INVALID_ARGUMENT,
                     "positions span more than 1e4 cell sizes from the origin; centre the object");
@@ -212,6 +217,8 @@ struct lh_pairsum {
   int precision = 0;
   lh_c64 delta{0.0, 0.0};
   int has_sub = 0;
+  int has_lat = 0;
+  int bloch_zero = 1;
 };
 
 struct lh_batch {
@@ -343,6 +350,9 @@ int lh_pairsum_create(lh_backend* b, const lh_pairsum_spec* spec, lh_pairsum** o
     s->precision = spec->precision;
     s->has_sub = spec->has_substrate ? 1 : 0;
     s->delta = spec->delta_eps;
+    s->has_lat = spec->has_lattice ? 1 : 0;
+    s->bloch_zero = (spec->k_bloch[0] == 0.0 && spec->k_bloch[1] == 0.0 &&
+                     spec->k_bloch[2] == 0.0) ? 1 : 0;
     LH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     std::vector<lh::Geometry> geoms;
     geoms.push_back(make_geometry(spec));
@@ -451,6 +461,63 @@ int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, dou
   });
 }
 
+int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, size_t max_iter,
+                            const lh_c64* dainv_deps, int dainv_uniform, double cross_sections[3],
+                            double* grad_eps, double* grad_pos, lh_solve_info* fwd_info,
+                            lh_solve_info* adj_info) {
+  return guarded([&]() {
+    if (!s || !inc || !dainv_deps || !cross_sections || !grad_eps)
+      return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+    if (s->has_lat && !s->bloch_zero)
+      return fail(LH_UNAVAILABLE,
+                  "the adjoint solve needs a complex-symmetric operator (Bloch k = 0)");
+    if (grad_pos && (s->has_sub || s->has_lat))
+      return fail(LH_UNAVAILABLE, "position gradients are built for free space only");
+    LH_HIP(hipSetDevice(s->b->device));
+    const bool mixed = s->precision == LH_PREC_MIXED;
+    const size_t n3 = 3 * s->n;
+    // forward: A p = E
+    s->eng->fill_incident(0, *inc);
+    s->eng->solve(tol, (long long)max_iter, mixed, true);
+    const auto r1 = s->eng->results();
+    fill_info(r1[0], fwd_info);
+    const auto cs = s->eng->cross_sections({inc->amplitude});
+    for (int q = 0; q < 3; ++q) cross_sections[q] = cs[q];
+    std::vector<lh_c64> p(n3), e(n3);
+    s->eng->download_slot(0, LH_SLOT_X, p.data());
+    s->eng->download_slot(0, LH_SLOT_E, e.data());
+    // adjoint: A lam = conj(E) (A = A^T)
+    std::vector<lh_c64> ec(n3);
+    for (size_t q = 0; q < n3; ++q) ec[q] = lh_c64{e[q].re, -e[q].im};
+    s->eng->upload_slot(0, LH_SLOT_RHS, ec.data());
+    s->eng->solve(tol, (long long)max_iter, mixed, true);
+    const auto r2 = s->eng->results();
+    fill_info(r2[0], adj_info);
+    // gradients with p in Krylov slot 0 (free after the solves)
+    s->eng->upload_slot(0, 0, p.data());
+    std::vector<lh_c64> dv(dainv_deps, dainv_deps + (dainv_uniform ? 9 : 9 * s->n));
+    std::vector<double> out;
+    s->eng->gradients(0, inc->direction, inc->amplitude, dv, dainv_uniform ? 1 : 0,
+                      grad_pos != nullptr, out);
+    for (size_t i = 0; i < s->n; ++i) {
+      grad_eps[2 * i] = out[5 * i];
+      grad_eps[2 * i + 1] = out[5 * i + 1];
+      if (grad_pos)
+        for (int c = 0; c < 3; ++c) grad_pos[3 * i + c] = out[5 * i + 2 + c];
+    }
+    // leave the session as after a forward solve
+    s->eng->upload_slot(0, LH_SLOT_X, p.data());
+    s->eng->upload_slot(0, LH_SLOT_RHS, e.data());
+    if (r1[0].status != LH_OK || r2[0].status != LH_OK)
+      return fail(LH_CONVERGENCE_FAILURE,
+                  std::string("GMRES did not converge within ") + std::to_string(max_iter) +
+                      " iterations (" + (r1[0].status != LH_OK ? "forward" : "adjoint") +
+                      " solve)");
+    return (int)LH_OK;
+  });
+}
+
 void lh_pairsum_destroy(lh_pairsum* s) {
   if (!s) return;
   (void)hipSetDevice(s->b->device);
@@ -473,6 +540,7 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
       int fcd;
       double cell;
       int lat;
+      const double* cls;
     };
     std::vector<Key> keys;
     std::vector<lh::Geometry> geoms;
@@ -489,14 +557,15 @@ int lh_batch_create(lh_backend* b, const lh_batch_problem* problems, size_t coun
       for (size_t t = 0; t < keys.size(); ++t)
         if (keys[t].p == sp.positions && keys[t].n == sp.n_dipoles &&
             keys[t].fcd == (sp.use_fcd ? 1 : 0) && keys[t].cell == sp.cell_size &&
-            keys[t].lat == (sp.has_lattice ? 1 : 0)) {
+            keys[t].lat == (sp.has_lattice ? 1 : 0) && keys[t].cls == sp.classify_positions) {
           gi = (int)t;
           break;
         }
       if (gi < 0) {
         gi = (int)geoms.size();
         keys.push_back(
-            Key{sp.positions, sp.n_dipoles, sp.use_fcd ? 1 : 0, sp.cell_size, sp.has_lattice ? 1 : 0});
+            Key{sp.positions, sp.n_dipoles, sp.use_fcd ? 1 : 0, sp.cell_size,
+                sp.has_lattice ? 1 : 0, sp.classify_positions});
         geoms.push_back(make_geometry(&sp));
       }
       probs.push_back(make_problem(&sp, gi));

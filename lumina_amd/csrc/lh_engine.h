@@ -36,6 +36,7 @@ struct Geometry {
   double cell = 0.0;
   float rskip2f = 0.0f;        // exclusion radius^2 on FP32 positions (0: no near list)
   std::vector<double> x, y, z;
+  std::vector<double> cls;     // [n][3] positions for the FCD decision, or empty
   int kw = 0;                  // ELL width (largest near count of a dipole)
   std::vector<int> ell_col;    // [kw][n] source index j
   std::vector<int> ell_tid;    // [kw][n] displacement-table id, -1 = pad (row end)
@@ -59,7 +60,7 @@ void detect_grid(Geometry& g);
 // satisfy r2f_exact <= rskip2f, i.e. exactly the pairs the far-field kernels
 // skip. Each pair points to a table of distinct displacements r_i - r_j
 // (FP64), flagged FCD when sqrt(|r_i - r_j|^2) <= 2 cell_size, the
-// reference's own test (greens.rs:102-106).
+// reference's own test (greens.rs:102-106), evaluated on g.cls when given.
 void build_near_list(Geometry& g);
 
 struct Problem {
@@ -126,6 +127,15 @@ class Engine {
   std::vector<double> cross_sections(const std::vector<double>& amplitude);
 
   double pairs_per_matvec(int s) const;
+
+  // Adjoint C_ext gradients of every solve (lh_adjoint.hip), with the forward
+  // moments in slot p_slot, the adjoint solution in X and the incident field
+  // in E: 5 doubles per dipole (dC/dRe eps, dC/dIm eps, dC/dr x y z), in
+  // solve order. dainv: d alpha^-1 / d eps blocks (9 or 9n complex) of
+  // solve 0's layout. positions = false leaves dC/dr at zero.
+  void gradients(int p_slot, const double dir[3], double amplitude,
+                 const std::vector<lh_c64>& dainv, int dainv_uniform, bool positions,
+                 std::vector<double>& out);
   hipStream_t stream() const { return stream_; }
 
   // Far-field launch timing (HIP events on the engine's stream).
@@ -174,6 +184,8 @@ class Engine {
   std::vector<FarItem> far_ewt_[2], far_ewp_[2];
   DevMem d_far_ewt_[2], d_far_ewp_[2];
   DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
+  DevMem d_gtab_, d_dainv_, d_gout_;   // adjoint: near gradient tensors, D blocks, output
+  long long n_tab_total_ = 0;
   int n_ewitems_ = 0;
   std::vector<int> ewl_cap_;   // records reserved per solve
   std::vector<double> ewl_stage_;   // host staging of a set_problem record update

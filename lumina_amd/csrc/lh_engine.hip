@@ -52,6 +52,16 @@ void launch_ewt32(hipStream_t, int, const FarItem*, const SolveDev*, const GStat
 void launch_ewp(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                 const Src64*, const double*, double2*, int);
 
+void launch_near_grad_tensors(hipStream_t, int, const SolveDev*, const int*, const int*,
+                              const double*, double*);
+void launch_prep_grad(hipStream_t, int, const VecItem*, const SolveDev*, const double*,
+                      const double*, const double*, const double2*, int, Src64*);
+void launch_grad_far(hipStream_t, int, const FarItem*, const SolveDev*, const Src64*, double2*,
+                     int);
+void launch_grad_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const int*,
+                        const int*, const double*, const double2*, int, const double2*,
+                        const double2*, int, long long, const double[3], double, int, double*);
+
 void check_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return;
   const int st = (e == hipErrorOutOfMemory) ? LH_OUT_OF_MEMORY : LH_DEVICE_ERROR;
@@ -99,12 +109,15 @@ static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
 namespace {
 struct DispKey {
   uint64_t a, b, c;
-  bool operator==(const DispKey& o) const { return a == o.a && b == o.b && c == o.c; }
+  int fcd;
+  bool operator==(const DispKey& o) const {
+    return a == o.a && b == o.b && c == o.c && fcd == o.fcd;
+  }
 };
 struct DispHash {
   size_t operator()(const DispKey& k) const {
     return (size_t)(k.a * 0x9E3779B97F4A7C15ULL ^ (k.b + 0x7F4A7C15ULL) * 0xC2B2AE3D27D4EB4FULL ^
-                    (k.c + 0x165667B1ULL) * 0x27D4EB2F165667C5ULL);
+                    (k.c + 0x165667B1ULL) * 0x27D4EB2F165667C5ULL ^ (uint64_t)k.fcd);
   }
 };
 }  // namespace
@@ -173,16 +186,24 @@ void build_near_list(Geometry& g) {
     for (size_t kk = 0; kk < rows[i].size(); ++kk) {
       const int j = rows[i][kk];
       const double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j], dz = g.z[i] - g.z[j];
+      // greens.rs:102-106: dist = sqrt(dx^2 + dy^2 + dz^2); FCD unless dist > 2 cell,
+      // on the classification positions when they are given
+      double dist;
+      if (g.cls.empty()) {
+        dist = std::sqrt(r2_exact(dx, dy, dz));
+      } else {
+        const double* ci = &g.cls[3 * (size_t)i];
+        const double* cj = &g.cls[3 * (size_t)j];
+        dist = std::sqrt(r2_exact(ci[0] - cj[0], ci[1] - cj[1], ci[2] - cj[2]));
+      }
+      const bool fcd = g.use_fcd && !(dist > 2.0 * g.cell);
       DispKey dk{__builtin_bit_cast(uint64_t, dx), __builtin_bit_cast(uint64_t, dy),
-                 __builtin_bit_cast(uint64_t, dz)};
+                 __builtin_bit_cast(uint64_t, dz), fcd ? 1 : 0};
       auto it = tab.find(dk);
       int u;
       if (it == tab.end()) {
         u = g.ntab++;
         tab.emplace(dk, u);
-        // greens.rs:102-106: dist = sqrt(dx^2 + dy^2 + dz^2); FCD unless dist > 2 cell
-        const double dist = std::sqrt(r2_exact(dx, dy, dz));
-        const bool fcd = g.use_fcd && !(dist > 2.0 * g.cell);
         g.disp.push_back(dx);
         g.disp.push_back(dy);
         g.disp.push_back(dz);
@@ -462,6 +483,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   }
   n_titems_ = (int)tis.size();
   n_ewitems_ = (int)ewis.size();
+  n_tab_total_ = tab_off;
 
   hipStream_t st = stream_;
   upload(d_px_, px, st);
@@ -846,6 +868,41 @@ void Engine::enqueue_cycle(bool mixed, bool early_exit) {
   matvec(1, LH_SLOT_X, LH_SLOT_R, 1, false);
   launch_zero(st, d_counter_.as<int>());
   launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, d_counter_.as<int>());
+}
+
+void Engine::gradients(int p_slot, const double dir[3], double amplitude,
+                       const std::vector<lh_c64>& dainv, int dainv_uniform, bool positions,
+                       std::vector<double>& out) {
+  hipStream_t st = stream_;
+  const int nv = (int)vit_.size();
+  long long ntot = 0;
+  for (const auto& d : sd_) ntot += d.n;
+  upload(d_dainv_, dainv, st);
+  const SolveDev* sd = d_solves_.as<SolveDev>();
+  if (positions) {
+    const size_t gbytes = sizeof(double) * 36 * (size_t)std::max<long long>(n_tab_total_, 1);
+    if (d_gtab_.bytes < gbytes) d_gtab_.alloc(gbytes);
+    launch_near_grad_tensors(st, n_titems_, sd, d_tis_.as<int>(), d_tiu_.as<int>(),
+                             d_disp_.as<double>(), d_gtab_.as<double>());
+    launch_prep_grad(st, nv, d_vit_.as<VecItem>(), sd, d_px_.as<double>(), d_py_.as<double>(),
+                     d_pz_.as<double>(), d_vec_.as<double2>(), p_slot, d_src64_.as<Src64>());
+    const int blk = tiling_.block[0] > 0 ? tiling_.block[0] : LH_TILE;
+    launch_grad_far(st, (int)far_free_[0].size(), d_far_free_[0].as<FarItem>(), sd,
+                    d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+  }
+  const size_t obytes = sizeof(double) * 5 * (size_t)std::max<long long>(ntot, 1);
+  if (d_gout_.bytes < obytes) d_gout_.alloc(obytes);
+  const double K = sd_[0].k / (amplitude * amplitude);
+  launch_grad_reduce(st, nv, d_vit_.as<VecItem>(), sd, d_ell_col_.as<int>(),
+                     d_ell_tid_.as<int>(), d_gtab_.p ? d_gtab_.as<double>() : nullptr,
+                     d_vec_.as<double2>(), p_slot, d_part_.as<double2>(),
+                     d_dainv_.as<double2>(), dainv_uniform, 0, dir, K, positions ? 1 : 0,
+                     d_gout_.as<double>());
+  LH_HIP(hipGetLastError());
+  out.resize(5 * (size_t)ntot);
+  LH_HIP(hipMemcpyAsync(out.data(), d_gout_.p, sizeof(double) * out.size(),
+                        hipMemcpyDeviceToHost, st));
+  LH_HIP(hipStreamSynchronize(st));
 }
 
 std::vector<SolveResult> Engine::results() {

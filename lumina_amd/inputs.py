@@ -414,6 +414,32 @@ def polarisability_block(material_name: str, wavelength_nm: float, spacing: floa
     return ellipsoid_tensor(vol, depol, eps, eps_m, k)
 
 
+def polarisability_from_epsilon(eps: complex, wavelength_nm: float, spacing: float,
+                                n_env: float, depol=None) -> np.ndarray:
+    """The alpha block of a dipole of permittivity eps (runner.rs:171-183)."""
+    k = wavenumber(wavelength_nm, n_env)
+    eps_m = n_env * n_env
+    vol = spacing ** 3
+    if depol is None:
+        return isotropic_block(radiative_correction(clausius_mossotti(vol, eps, eps_m), k))
+    return ellipsoid_tensor(vol, depol, eps, eps_m, k)
+
+
+def inverse_polarisability_derivative(eps: complex, spacing: float, n_env: float) -> np.ndarray:
+    """d(alpha^-1)/d eps for the CM + radiative-reaction model (types.rs:243-259,
+    286-317), as a row-major 3x3 block (no reference counterpart).
+
+    alpha_RR^-1 = alpha^-1 - i k^3 / (6 pi) per principal axis, and
+    alpha^-1 = (eps_m + L (eps - eps_m)) / (V (eps - eps_m)) (L = 1/3 for the
+    isotropic CM form), so d(alpha^-1)/d eps = -eps_m / (V (eps - eps_m)^2) on
+    every axis: a multiple of the identity for any depolarisation factors
+    and orientation."""
+    eps_m = n_env * n_env
+    vol = spacing ** 3
+    d = -eps_m / (vol * (complex(eps) - eps_m) ** 2)
+    return isotropic_block(d)
+
+
 # ---------------------------------------------------------------------------
 # Workloads named by what they are
 # ---------------------------------------------------------------------------

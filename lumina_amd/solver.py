@@ -127,7 +127,7 @@ def wavenumber(wavelength_nm: float, n_medium: float) -> float:
 
 
 def _spec(positions, polarisability, k, use_fcd, cell_size, substrate, precision, lattice=None,
-          k_bloch=(0.0, 0.0, 0.0)):
+          k_bloch=(0.0, 0.0, 0.0), classify_positions=None):
     pos = _f64(positions).reshape(-1, 3)
     pol = _c128(polarisability)
     n = pos.shape[0]
@@ -164,8 +164,14 @@ def _spec(positions, polarisability, k, use_fcd, cell_size, substrate, precision
         d = complex(substrate.delta_eps)
         sp.delta_eps = _lib.c64(d.real, d.imag)
     sp.precision = int(precision)
+    cls = None
+    if classify_positions is not None:
+        cls = _f64(classify_positions).reshape(-1, 3)
+        if cls.shape != pos.shape:
+            raise ValueError("classify_positions must have the shape of positions")
+        sp.classify_positions = dptr(cls)
     # keep the arrays alive as long as the spec
-    return sp, (pos, pol)
+    return sp, (pos, pol, cls)
 
 
 class MatvecSession:
@@ -205,10 +211,11 @@ class PairSumSession:
     def __init__(self, backend: "HipBackend", positions, polarisability, k: float,
                  use_fcd: bool = True, cell_size: float = 3.0,
                  substrate: SubstrateRuntime | None = None, precision: int = FP64,
-                 lattice: LatticeSpec | None = None, k_bloch=(0.0, 0.0, 0.0)):
+                 lattice: LatticeSpec | None = None, k_bloch=(0.0, 0.0, 0.0),
+                 classify_positions=None):
         self._lib = backend._lib
         sp, keep = _spec(positions, polarisability, k, use_fcd, cell_size, substrate, precision,
-                         lattice, k_bloch)
+                         lattice, k_bloch, classify_positions)
         self.n = int(sp.n_dipoles)
         self.k = float(k)
         self.precision = int(precision)
@@ -276,6 +283,34 @@ class PairSumSession:
             raise e
         check(st)
         return out, p, info.as_dict()
+
+    def adjoint_grad(self, inc: IncidentField, tolerance: float, max_iterations: int, dainv,
+                     positions: bool = True):
+        """C_ext and its gradients through the forward and adjoint solves
+        (lh_pairsum_adjoint_grad). ``dainv``: d(alpha^-1)/d eps, (9,) or (N, 9).
+        Returns (cross_sections[3], grad_eps (N, 2) = dC/dRe eps, dC/dIm eps,
+        grad_pos (N, 3) or None, forward info, adjoint info)."""
+        dv = _c128(dainv).reshape(-1)
+        uniform = 1 if dv.size == 9 else 0
+        if not uniform and dv.size != 9 * self.n:
+            raise ValueError("dainv must be (9,) or (N, 9)")
+        out = np.zeros(3)
+        ge = np.zeros(2 * self.n)
+        gp = np.zeros(3 * self.n) if positions else None
+        fi, ai = _lib.SolveInfo(), _lib.SolveInfo()
+        ci = inc._c()
+        st = self._lib.lh_pairsum_adjoint_grad(self._h, C.byref(ci), float(tolerance),
+                                               int(max_iterations), cptr(dv), uniform, dptr(out),
+                                               dptr(ge), dptr(gp) if positions else None,
+                                               C.byref(fi), C.byref(ai))
+        if st == _lib.LH_CONVERGENCE_FAILURE:
+            e = ConvergenceFailure(st, self._lib.lh_last_error().decode())
+            e.info = fi.as_dict()
+            e.adjoint_info = ai.as_dict()
+            raise e
+        check(st)
+        return (out, ge.reshape(-1, 2), gp.reshape(-1, 3) if positions else None,
+                fi.as_dict(), ai.as_dict())
 
     def close(self):
         if getattr(self, "_h", None):

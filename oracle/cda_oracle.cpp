@@ -128,6 +128,8 @@ void greens_at(const double s[3], double k, T33 g) {
 // Gauss-Legendre average over the source cell when dist <= 2*cell_size).
 // Pinned by: mpmath quadrature golden values (tests/golden/greens_fcd.json),
 // greens.rs:201-261 (FCD -> point G at 10 cells; finite and different at R=d).
+void greens_fcd_average(const double ro[3], const double rs[3], double k, double cell_size,
+                        T33 out);
 void greens_filtered(const double ro[3], const double rs[3], double k,
                      double cell_size, T33 out) {
   double dx = ro[0] - rs[0];
@@ -138,6 +140,12 @@ void greens_filtered(const double ro[3], const double rs[3], double k,
     greens_point(ro, rs, k, out);
     return;
   }
+  greens_fcd_average(ro, rs, k, cell_size, out);
+}
+
+// The 27-node average of greens.rs:120-164 (the branch dist <= 2 cell_size).
+void greens_fcd_average(const double ro[3], const double rs[3], double k, double cell_size,
+                        T33 out) {
   const double nodes[3] = {-std::sqrt(3.0 / 5.0), 0.0, std::sqrt(3.0 / 5.0)};
   const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
   double half_d = cell_size / 2.0;
@@ -350,7 +358,111 @@ struct Problem {
   int has_sub;
   double z_interface;
   C64 delta;
+  // Finite-difference support (no reference counterpart): when set, the FCD
+  // near/far decision of pair (i, j) is taken on these positions instead of
+  // pos, so that a perturbed dipole keeps its unperturbed classification.
+  const double* cls_pos = nullptr;
 };
+
+// The direct tensor of pair (i, j) on the free-space path: FCD filtered or
+// point (assembly.rs:252-261), with the classification of cls_pos if set.
+void direct_tensor(const Problem& P, int64_t i, int64_t j, T33 g) {
+  const double* ri = P.pos + 3 * i;
+  const double* rj = P.pos + 3 * j;
+  if (!P.use_fcd) {
+    greens_point(ri, rj, P.k, g);
+    return;
+  }
+  if (!P.cls_pos) {
+    greens_filtered(ri, rj, P.k, P.cell_size, g);
+    return;
+  }
+  const double* ci = P.cls_pos + 3 * i;
+  const double* cj = P.cls_pos + 3 * j;
+  double dx = ci[0] - cj[0], dy = ci[1] - cj[1], dz = ci[2] - cj[2];
+  double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+  if (dist > 2.0 * P.cell_size) greens_point(ri, rj, P.k, g);
+  else greens_fcd_average(ri, rj, P.k, P.cell_size, g);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gradients (no reference counterpart; the reference has no
+// derivatives). Derived here independently of the device code: the product
+// rule on G_ab = pf (a delta_ab + b rh_a rh_b) of greens.rs:34-71 with
+// pf(r), a(r), b(r) and d rh_a / d r_c = (delta_ac - rh_a rh_c) / r.
+// Pinned by: central finite differences of C_ext on this oracle's own
+// solves (tests/test_oracle.py) and on the device solves.
+// ---------------------------------------------------------------------------
+
+typedef C64 T333[3][3][3];   // [c][a][b] = d G_ab / d (r_obs)_c
+
+void greens_point_grad(const double ro[3], const double rs[3], double k, T333 dg) {
+  double rx = ro[0] - rs[0], ry = ro[1] - rs[1], rz = ro[2] - rs[2];
+  double r_sq = rx * rx + ry * ry + rz * rz;
+  double r = std::sqrt(r_sq);
+  double kr = k * r;
+  double kr_sq = kr * kr;
+  C64 ikr = c(0.0, kr);
+  C64 pf = divs(muls(cexp(ikr), k * k), 4.0 * PI * r);
+  C64 a = add(c(1.0), cdiv(sub(ikr, c(1.0)), c(kr_sq)));
+  C64 b = cdiv(sub(sub(c(3.0), mul(c(3.0), ikr)), c(kr_sq)), c(kr_sq));
+  C64 dpf = mul(pf, c(-1.0 / r, k));                                      // pf (ik - 1/r)
+  C64 da = c(2.0 / (k * k * r * r * r), -1.0 / (k * r * r));             // d a / dr
+  C64 db = c(-6.0 / (k * k * r * r * r), 3.0 / (k * r * r));             // d b / dr
+  double rh[3] = {rx / r, ry / r, rz / r};
+  C64 pfb = mul(pf, b);
+  for (int cc = 0; cc < 3; ++cc)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double dij = (i == j) ? 1.0 : 0.0;
+        double dic = (i == cc) ? 1.0 : 0.0;
+        double djc = (j == cc) ? 1.0 : 0.0;
+        double hh = rh[i] * rh[j];
+        C64 radial = add(mul(dpf, add(muls(a, dij), muls(b, hh))),
+                         mul(pf, add(muls(da, dij), muls(db, hh))));
+        double ang = ((dic - rh[i] * rh[cc]) * rh[j] + rh[i] * (djc - rh[j] * rh[cc])) / r;
+        dg[cc][i][j] = add(muls(radial, rh[cc]), muls(pfb, ang));
+      }
+}
+
+// Gradient of the 27-node FCD average (greens.rs:120-164), with the same
+// node skip and weight normalisation.
+void greens_fcd_average_grad(const double ro[3], const double rs[3], double k, double cell_size,
+                             T333 out) {
+  const double nodes[3] = {-std::sqrt(3.0 / 5.0), 0.0, std::sqrt(3.0 / 5.0)};
+  const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  double half_d = cell_size / 2.0;
+  T333 acc;
+  for (int q = 0; q < 27; ++q) (&acc[0][0][0])[q] = c(0.0);
+  double total_w = 0.0;
+  for (int ix = 0; ix < 3; ++ix)
+    for (int iy = 0; iy < 3; ++iy)
+      for (int iz = 0; iz < 3; ++iz) {
+        double rp[3] = {rs[0] + half_d * nodes[ix], rs[1] + half_d * nodes[iy],
+                        rs[2] + half_d * nodes[iz]};
+        double ddx = ro[0] - rp[0], ddy = ro[1] - rp[1], ddz = ro[2] - rp[2];
+        if (ddx * ddx + ddy * ddy + ddz * ddz < 1e-20) continue;
+        double w = wts[ix] * wts[iy] * wts[iz];
+        T333 g;
+        greens_point_grad(ro, rp, k, g);
+        for (int q = 0; q < 27; ++q) (&acc[0][0][0])[q] = add((&acc[0][0][0])[q], muls((&g[0][0][0])[q], w));
+        total_w += w;
+      }
+  for (int q = 0; q < 27; ++q)
+    (&out[0][0][0])[q] = total_w > 1e-30 ? divs((&acc[0][0][0])[q], total_w) : (&acc[0][0][0])[q];
+}
+
+// Gradient of direct_tensor with respect to the observation position r_i.
+void direct_tensor_grad(const Problem& P, int64_t i, int64_t j, T333 dg) {
+  const double* ri = P.pos + 3 * i;
+  const double* rj = P.pos + 3 * j;
+  const double* ci = P.cls_pos ? P.cls_pos + 3 * i : ri;
+  const double* cj = P.cls_pos ? P.cls_pos + 3 * j : rj;
+  double dx = ci[0] - cj[0], dy = ci[1] - cj[1], dz = ci[2] - cj[2];
+  double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+  if (!P.use_fcd || dist > 2.0 * P.cell_size) greens_point_grad(ri, rj, P.k, dg);
+  else greens_fcd_average_grad(ri, rj, P.k, P.cell_size, dg);
+}
 
 // assembly.rs:209-318 matvec_on_the_fly, one row block i.
 void matvec_row(const Problem& P, const C64* x, int64_t i, C64 yi[3]) {
@@ -368,10 +480,8 @@ void matvec_row(const Problem& P, const C64* x, int64_t i, C64 yi[3]) {
       if (P.lattice) {
         double r[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
         ewald_eval(*P.lattice, r, P.kb, P.k, g);
-      } else if (P.use_fcd) {
-        greens_filtered(ri, rj, P.k, P.cell_size, g);
       } else {
-        greens_point(ri, rj, P.k, g);
+        direct_tensor(P, i, j, g);
       }
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) yi[a] = sub(yi[a], mul(g[a][b], x[3 * j + b]));
@@ -670,6 +780,7 @@ struct OrSpec {
   int has_sub;
   double z_interface;
   double delta[2];
+  const double* cls_pos;   // FCD classification positions or null
 };
 
 static Problem to_problem(const OrSpec* s, Lattice2D* L) {
@@ -691,6 +802,7 @@ static Problem to_problem(const OrSpec* s, Lattice2D* L) {
   P.has_sub = s->has_sub;
   P.z_interface = s->z_interface;
   P.delta = c(s->delta[0], s->delta[1]);
+  P.cls_pos = s->cls_pos;
   return P;
 }
 
@@ -736,10 +848,8 @@ void or_assemble(const OrSpec* spec, double* out) {
         if (P.lattice) {
           double r[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
           ewald_eval(*P.lattice, r, P.kb, P.k, g);
-        } else if (P.use_fcd) {
-          greens_filtered(ri, rj, P.k, P.cell_size, g);
         } else {
-          greens_point(ri, rj, P.k, g);
+          direct_tensor(P, i, j, g);
         }
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) {
@@ -855,6 +965,73 @@ void or_cross_sections(int64_t n, const double* alpha, int alpha_uniform, const 
   out3[0] = cext;
   out3[1] = cabs;
   out3[2] = std::max(cext - cabs, 0.0);
+}
+
+// Adjoint gradients of C_ext = (k/E0^2) Im(E^H p), A p = E, A symmetric,
+// A lam = conj(E) (no reference counterpart):
+//   grad_eps[i] = (dC/dRe eps_i, dC/dIm eps_i) = (k/E0^2) (Im f, Re f),
+//     f = -lam_i^T D_i p_i, D_i = d(alpha_i^-1)/d eps_i (caller's model);
+//   grad_r[i][c] = (k/E0^2) Im( -i k dir_c conj(E_i).p_i + i k dir_c lam_i.E_i
+//     + sum_{j != i} [lam_i^T dG_c(i,j) p_j + p_i^T dG_c(i,j) lam_j] ),
+//   dG_c(i,j) = d G(r_i, r_j) / d r_i,c (FCD classification frozen as in
+//   direct_tensor). Positions need free space: returns 1 (grad_r zero) with
+//   a lattice or a substrate.
+int or_adjoint_grad(const OrSpec* spec, const double* dir, const double* pol6, double amplitude,
+                    const double* p, const double* lam, const double* dainv, int dainv_uniform,
+                    double* grad_eps, double* grad_r, const int64_t* rows, int64_t nrows) {
+  Lattice2D L;
+  Problem P = to_problem(spec, &L);
+  const int64_t n = P.n;
+  const C64* Pv = reinterpret_cast<const C64*>(p);
+  const C64* Lv = reinterpret_cast<const C64*>(lam);
+  const C64* D = reinterpret_cast<const C64*>(dainv);
+  const double K = P.k / (amplitude * amplitude);
+  std::vector<C64> E(3 * n);
+  or_incident(n, P.pos, dir, pol6, amplitude, P.k, reinterpret_cast<double*>(E.data()));
+  for (int64_t i = 0; i < n; ++i) {
+    const C64* Di = dainv_uniform ? D : D + 9 * i;
+    C64 f = c(0.0);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) f = add(f, mul(mul(Lv[3 * i + a], Di[3 * a + b]), Pv[3 * i + b]));
+    f = neg(f);
+    grad_eps[2 * i] = K * f.im;
+    grad_eps[2 * i + 1] = K * f.re;
+  }
+  std::fill(grad_r, grad_r + 3 * n, 0.0);
+  if (P.lattice || P.has_sub) return 1;
+  // rows: the dipoles whose position gradient is wanted (all when null)
+  const int64_t nr = rows ? nrows : n;
+  for (int64_t q = 0; q < nr; ++q) {
+    const int64_t i = rows ? rows[q] : q;
+    C64 t1 = c(0.0), t2 = c(0.0);
+    for (int a = 0; a < 3; ++a) {
+      t1 = add(t1, mul(conj(E[3 * i + a]), Pv[3 * i + a]));
+      t2 = add(t2, mul(Lv[3 * i + a], E[3 * i + a]));
+    }
+    C64 g[3];
+    for (int cc = 0; cc < 3; ++cc)
+      g[cc] = add(mul(c(0.0, -P.k * dir[cc]), t1), mul(c(0.0, P.k * dir[cc]), t2));
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      T333 dg;
+      direct_tensor_grad(P, i, j, dg);
+      for (int cc = 0; cc < 3; ++cc)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            g[cc] = add(g[cc], mul(mul(Lv[3 * i + a], dg[cc][a][b]), Pv[3 * j + b]));
+            g[cc] = add(g[cc], mul(mul(Pv[3 * i + a], dg[cc][a][b]), Lv[3 * j + b]));
+          }
+    }
+    for (int cc = 0; cc < 3; ++cc) grad_r[3 * i + cc] = K * g[cc].im;
+  }
+  return 0;
+}
+
+// Point-tensor gradient d G_ab / d (r_o)_c as [c][a][b] complex.
+void or_greens_point_grad(const double* ro, const double* rs, double k, double* out) {
+  T333 g;
+  greens_point_grad(ro, rs, k, g);
+  std::memcpy(out, g, sizeof(g));
 }
 
 }  // extern "C"

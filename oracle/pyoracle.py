@@ -50,6 +50,10 @@ def load():
         _lib.or_solve_gmres_dense.restype = C.c_int
         _lib.or_incident.argtypes = [C.c_int64, D, D, D, C.c_double, C.c_double, D]
         _lib.or_cross_sections.argtypes = [C.c_int64, D, C.c_int, D, D, C.c_double, C.c_double, D]
+        _lib.or_adjoint_grad.argtypes = [C.c_void_p, D, D, C.c_double, D, D, D, C.c_int, D, D,
+                                         C.POINTER(C.c_int64), C.c_int64]
+        _lib.or_adjoint_grad.restype = C.c_int
+        _lib.or_greens_point_grad.argtypes = [D, D, C.c_double, D]
     return _lib
 
 
@@ -135,7 +139,7 @@ class _Spec(C.Structure):
         ("cell_size", C.c_double), ("has_lattice", C.c_int), ("a1", C.c_double * 3),
         ("a2", C.c_double * 3), ("eta", C.c_double), ("n_real", C.c_int), ("n_recip", C.c_int),
         ("kb", C.c_double * 3), ("has_sub", C.c_int), ("z_interface", C.c_double),
-        ("delta", C.c_double * 2),
+        ("delta", C.c_double * 2), ("cls_pos", C.POINTER(C.c_double)),
     ]
 
 
@@ -143,7 +147,7 @@ class Problem:
     """The arguments of matvec_on_the_fly (assembly.rs:209-218)."""
 
     def __init__(self, positions, polarisability, k, use_fcd=True, cell_size=3.0,
-                 substrate=None, lattice=None, k_bloch=(0.0, 0.0, 0.0)):
+                 substrate=None, lattice=None, k_bloch=(0.0, 0.0, 0.0), classify_positions=None):
         self.pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
         pol = np.ascontiguousarray(np.asarray(polarisability, dtype=np.complex128))
         self.n = self.pos.shape[0]
@@ -172,6 +176,11 @@ class Problem:
             s.z_interface = float(substrate[0])
             d = complex(substrate[1])
             s.delta[0], s.delta[1] = d.real, d.imag
+        if classify_positions is not None:
+            # FCD near/far decisions taken on these positions (finite differences)
+            self._cls = np.ascontiguousarray(np.asarray(classify_positions,
+                                                        dtype=np.float64).reshape(-1, 3))
+            s.cls_pos = _d(self._cls)
         self.spec = s
         self.k = float(k)
 
@@ -229,6 +238,39 @@ class Problem:
         e = self.incident(direction, polarisation, amplitude)
         p, info = self.solve_gmres(e, tol, max_iter, nthreads)
         return self.cross_sections(p, e, amplitude), p, info
+
+
+    def adjoint_grad(self, p, lam, dainv, direction=(0, 0, 1), polarisation=(1, 0, 0),
+                     amplitude=1.0, rows=None):
+        """C_ext gradients from the forward moments p and the adjoint
+        solution lam (A lam = conj(E)): (grad_eps (N, 2) as d/dRe, d/dIm;
+        grad_r (N, 3) or None without free space). ``rows`` limits the
+        position gradients to those dipoles (the others are left at 0)."""
+        pv = np.ascontiguousarray(np.asarray(p, dtype=np.complex128).reshape(-1))
+        lv = np.ascontiguousarray(np.asarray(lam, dtype=np.complex128).reshape(-1))
+        dv = np.ascontiguousarray(np.asarray(dainv, dtype=np.complex128).reshape(-1))
+        uniform = 1 if dv.size == 9 else 0
+        pol = np.array([[complex(q).real, complex(q).imag] for q in polarisation]).reshape(6)
+        ge = np.zeros(2 * self.n)
+        gr = np.zeros(3 * self.n)
+        if rows is None:
+            rp, nrows = None, 0
+        else:
+            self._rows = np.ascontiguousarray(np.asarray(rows, dtype=np.int64))
+            rp, nrows = self._rows.ctypes.data_as(C.POINTER(C.c_int64)), len(self._rows)
+        st = load().or_adjoint_grad(C.byref(self.spec), _d(_v3(direction)),
+                                    _d(np.ascontiguousarray(pol)), float(amplitude),
+                                    _d(pv.view(np.float64)), _d(lv.view(np.float64)),
+                                    _d(dv.view(np.float64)), uniform, _d(ge), _d(gr),
+                                    rp, nrows)
+        return ge.reshape(-1, 2), (gr.reshape(-1, 3) if st == 0 else None)
+
+
+def greens_point_grad(ro, rs, k):
+    """d G_ab / d (r_o)_c as a (3, 3, 3) complex array [c, a, b]."""
+    out = np.zeros(54)
+    load().or_greens_point_grad(_d(_v3(ro)), _d(_v3(rs)), float(k), _d(out))
+    return out.view(np.complex128).reshape(3, 3, 3)
 
 
 def solve_gmres_dense(A, rhs, tol, max_iter):
