@@ -19,7 +19,14 @@ Other workloads: --workload cuboid (80x40x40 nm Au cuboid, 1.5 nm spacing,
 38,637 dipoles, 520 nm, one solve FP64 and mixed); --workload metasurface
 (Au cylinder D=100 nm H=40 nm at 2 nm spacing, 41,181 dipoles, on glass with
 the retarded Fresnel image, 300 nm square lattice with 2D Ewald sums, Bloch
-k = 0, a batch of wavelengths in 500..890 nm, FP64 and mixed).
+k = 0, a batch of wavelengths in 500..890 nm, FP64 and mixed); --workload
+sphere (Au sphere r=10 nm, d=2 nm, 515 dipoles, water, 31 wavelengths
+400..700 nm, FP64 GMRES tol 1e-10, one batched solve, compared with the
+oracle); --workload helix (Au helix, coil 25 nm, pitch 40 nm, 4 turns, wire
+8 nm, d=1.5 nm, 35,013 dipoles, water, 600 nm, left and right circular
+light: forward + adjoint solves and the dC_ext/d(eps_i, r_i) gradients,
+checked against the oracle's CPU adjoint on sampled dipoles and against
+central differences).
 """
 from __future__ import annotations
 
@@ -63,15 +70,30 @@ def pairs(n):
     return n * (n - 1.0)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(probs, budget_s=20.0):
     """The oracle (C++ FP64 restatement of the reference CPU path: rows of
     the on-the-fly matvec over a thread pool, serial GMRES) on a bounded
-    sample: the batch's problems at the middle wavelength with N <= 2000,
-    smallest first, until the time budget is used."""
+    sample: full solves of the batch's problems at the middle wavelength with
+    N <= 2000, smallest first, until the time budget is used (the value), and
+    the matvec rate on the median-size problem with all threads and with one
+    (median of 3 after a warm-up)."""
     from oracle import pyoracle as O
     threads = min(16, os.cpu_count() or 1)
-    mid = sorted(set(p["wl"] for p in probs))[len(set(p["wl"] for p in probs)) // 2]
-    sample = sorted([p for p in probs if p["wl"] == mid and p["n"] <= 2000], key=lambda p: p["n"])
+    wls = sorted(set(p["wl"] for p in probs))
+    mid = wls[len(wls) // 2]
+    at_mid = sorted([p for p in probs if p["wl"] == mid], key=lambda p: p["n"])
+    sample = [p for p in at_mid if p["n"] <= 2000]
     t0 = time.perf_counter()
     ev, nsolve = 0.0, 0
     for p in sample:
@@ -82,11 +104,26 @@ def cpu_baseline(probs, budget_s=20.0):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    med = at_mid[len(at_mid) // 2]
+    P = O.Problem(med["positions"], med["polarisability"], med["k"], True, med["cell_size"])
+    x = np.random.default_rng(0).standard_normal(3 * med["n"]) + 0j
+    rates = {}
+    for nt in (threads, 1):
+        P.matvec(x, nthreads=nt)
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            P.matvec(x, nthreads=nt)
+            ts.append(time.perf_counter() - t1)
+        rates[nt] = pairs(med["n"]) / float(np.median(ts))
     return {"value": ev / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
-            "solves_per_s": nsolve / dt,
+            "cpu_model": cpu_model(), "solves_per_s": nsolve / dt,
+            "matvec_pair_evals_per_s": {"all_threads": rates[threads], "one_thread": rates[1],
+                                        "n": med["n"], "threads": threads},
             "sample": f"{nsolve} full GMRES solves (tol 1e-8, FP64, FCD) of the batch's "
                       f"problems at {mid:.1f} nm with N <= 2000, smallest first; "
-                      f"{ev:.3g} pair-evals in {dt:.1f} s"}
+                      f"{ev:.3g} pair-evals in {dt:.1f} s; matvec rates: median of 3 on the "
+                      f"N={med['n']} problem"}
 
 
 def results_allow_failures(batch):
@@ -338,12 +375,154 @@ def run_metasurface(args):
     }
 
 
+def run_sphere(args):
+    """31 wavelengths of the 515-dipole sphere in one FP64 batch (tol 1e-10),
+    every C_ext/C_abs/C_sca compared with the oracle's GMRES."""
+    from lumina_amd import inputs as I
+    from lumina_amd.solver import FP64, Batch, HipBackend
+    from oracle import pyoracle as O
+    pos = I.au_sphere_r10_d2()
+    n_env, d = 1.33, 2.0
+    wls = [400.0 + 10.0 * t for t in range(31)]
+    probs = [dict(positions=pos, polarisability=I.polarisability_block("Au", lam, d, n_env),
+                  k=I.wavenumber(lam, n_env), use_fcd=True, cell_size=d) for lam in wls]
+    be = HipBackend(0)
+    b = Batch(be, probs)
+    for _ in range(args.warmup):
+        b.run(1e-10, args.max_iter, FP64)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(1e-10, args.max_iter, FP64)
+    dt = (time.perf_counter() - t0) / args.steps
+    cs, infos = results_allow_failures(b)
+    b.close()
+    ev = sum(i["pair_evals"] for i in infos)
+    t1 = time.perf_counter()
+    errs, ev_cpu = [], 0.0
+    threads = min(16, os.cpu_count() or 1)
+    for q, lam in enumerate(wls):
+        P = O.Problem(pos, probs[q]["polarisability"], probs[q]["k"], True, d)
+        cso, _, io = P.compute_cross_sections(1e-10, args.max_iter, nthreads=threads)
+        ev_cpu += io["matvecs"] * pairs(len(pos))
+        if io["status"] == 0 and infos[q]["status"] == 0:
+            errs.append(float(np.max(np.abs(cs[q] - cso) / np.abs(cso))))
+    t_cpu = time.perf_counter() - t1
+    log(f"[bench] sphere spectrum: GPU {dt:.3f} s, oracle {t_cpu:.1f} s, max rel err "
+        f"{max(errs) if errs else None}")
+    return {
+        "metric": "Green pair-evals/s (31-wavelength FP64 spectrum, one batched solve)",
+        "value": ev / dt, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "Au sphere r=10 nm d=2 nm (515 dipoles), water, 31 wavelengths "
+                               "400..700 nm, FCD, GMRES(30) tol 1e-10", "parallelism": "single GPU"},
+        "solves_per_s": len(wls) / dt, "iterations": [i["iterations"] for i in infos],
+        "spectrum_nm": wls, "c_ext": cs[:, 0].tolist(), "c_abs": cs[:, 1].tolist(),
+        "c_sca": cs[:, 2].tolist(),
+        "max_rel_err_vs_oracle": max(errs) if errs else None,
+        "cpu_baseline": {"value": ev_cpu / t_cpu, "unit": "pair-evals/s", "cores": threads,
+                         "kind": "port", "sample": "the same 31 solves by the oracle (FP64 "
+                                                   "restatement, rows over a thread pool)"},
+    }
+
+
+def run_helix(args):
+    """Forward and adjoint solves of the 35,013-dipole helix under left and
+    right circular light, FP64 and mixed, with the gradient checks."""
+    from lumina_amd import inputs as I
+    from lumina_amd.solver import FP64, MIXED, HipBackend, IncidentField, PairSumSession
+    from oracle import pyoracle as O
+    lam, n_env, d = 600.0, 1.33, 1.5
+    pos = I.au_helix_r25_p40_t4_w8_d1p5()
+    n = len(pos)
+    k = I.wavenumber(lam, n_env)
+    eps = I.material("Au").epsilon(lam)
+    a = I.polarisability_block("Au", lam, d, n_env)
+    D = I.inverse_polarisability_derivative(eps, d, n_env)
+    tol = 1e-10
+    be = HipBackend(0)
+    pols = {"LCP": (1 / np.sqrt(2), 1j / np.sqrt(2), 0.0), "RCP": (1 / np.sqrt(2), -1j / np.sqrt(2), 0.0)}
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        s = PairSumSession(be, pos, a, k, True, d, precision=prec)
+        for _ in range(args.warmup):
+            s.adjoint_grad(IncidentField(polarisation=pols["LCP"]), tol, args.max_iter, D)
+        out = {}
+        t0 = time.perf_counter()
+        for hand, pol in pols.items():
+            cs, ge, gp, fi, ai = s.adjoint_grad(IncidentField(polarisation=pol), tol,
+                                                args.max_iter, D)
+            out[hand] = dict(cs=cs, ge=ge, gp=gp, fi=fi, ai=ai)
+        dt = time.perf_counter() - t0
+        ev = sum(o["fi"]["pair_evals"] + o["ai"]["pair_evals"] for o in out.values())
+        res[name] = dict(s=dt, ev=ev, out=out, session=s)
+        log(f"[bench] helix {name}: {dt:.2f} s for 2 x (forward + adjoint + gradients), "
+            f"iterations {[(o['fi']['iterations'], o['ai']['iterations']) for o in out.values()]}")
+    # checks on the LCP gradients: oracle CPU adjoint on 32 seeded dipoles
+    # (given the device's p and lam) and central differences of C_ext in Re eps
+    s = res["fp64"]["session"]
+    inc = IncidentField(polarisation=pols["LCP"])
+    g = res["fp64"]["out"]["LCP"]
+    _, p, _ = s.compute_cross_sections(inc, 1e-12, args.max_iter)
+    lv, _ = s.solve_gmres(np.conj(s.incident(inc)), 1e-12, args.max_iter)
+    cs12, ge12, gp12, _, _ = s.adjoint_grad(inc, 1e-12, args.max_iter, D)
+    rows = np.random.default_rng(1).choice(n, 32, replace=False)
+    t1 = time.perf_counter()
+    geo, gro = O.Problem(pos, a, k, True, d).adjoint_grad(p, lv, D, polarisation=pols["LCP"],
+                                                          rows=rows)
+    t_or = time.perf_counter() - t1
+    e_or = float(np.max(np.abs(gp12[rows] - gro[rows])) / np.max(np.abs(gro[rows])))
+    fd_err = 0.0
+    alphas = np.tile(a, (n, 1))
+    for i in np.random.default_rng(2).choice(n, args.fd_dipoles, replace=False):
+        h = 1e-6 * abs(eps)
+        c = []
+        for sg in (1, -1):
+            a2 = alphas.copy()
+            a2[i] = I.polarisability_from_epsilon(eps + sg * h, lam, d, n_env)
+            ss = PairSumSession(be, pos, a2, k, True, d)
+            c.append(ss.compute_cross_sections(inc, 1e-12, args.max_iter)[0][0])
+            ss.close()
+        fd = (c[0] - c[1]) / (2 * h)
+        fd_err = max(fd_err, abs(fd - ge12[i, 0]) / np.max(np.abs(ge12[:, 0])))
+    g64, g32 = res["fp64"]["out"], res["mixed"]["out"]
+    mixed_err = max(float(np.max(np.abs(g32[h]["gp"] - g64[h]["gp"])) / np.max(np.abs(g64[h]["gp"])))
+                    for h in pols)
+    cext_err = max(abs(g32[h]["cs"][0] - g64[h]["cs"][0]) / abs(g64[h]["cs"][0]) for h in pols)
+    for r in res.values():
+        r["session"].close()
+    log(f"[bench] helix checks: oracle rows {e_or:.2e} ({t_or:.1f} s), FD {fd_err:.2e}, "
+        f"mixed vs fp64 gradient {mixed_err:.2e}")
+    m = res["mixed"]
+    return {
+        "metric": "Green pair-evals/s (helix forward + adjoint solves, mixed precision)",
+        "value": m["ev"] / m["s"], "unit": "pair-evals/s", "n_gpus": 1, "steps": 1,
+        "warmup": args.warmup, "ms_per_step": 1e3 * m["s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 far field + fp64 Krylov",
+        "data": "synthetic",
+        "config": {"workload": f"Au helix coil 25 pitch 40 4 turns wire 8 d=1.5 ({n} dipoles), "
+                               f"water, 600 nm, LCP and RCP, forward + adjoint GMRES(30) tol "
+                               f"{tol:g} + gradients", "parallelism": "single GPU"},
+        "c_ext": {h: float(g64[h]["cs"][0]) for h in pols},
+        "circular_dichroism_ext": float(g64["LCP"]["cs"][0] - g64["RCP"]["cs"][0]),
+        "fp64": {"s": res["fp64"]["s"], "pair_evals_per_s": res["fp64"]["ev"] / res["fp64"]["s"],
+                 "iterations": {h: (o["fi"]["iterations"], o["ai"]["iterations"])
+                                for h, o in g64.items()}},
+        "mixed_iterations": {h: (o["fi"]["iterations"], o["ai"]["iterations"]) for h, o in g32.items()},
+        "cext_rel_err_mixed_vs_fp64": cext_err, "grad_rel_err_mixed_vs_fp64": mixed_err,
+        "grad_check": {"oracle_rows": 32, "oracle_rel_err": e_or, "fd_dipoles": args.fd_dipoles,
+                       "fd_rel_err": fd_err, "fd_step": "1e-6 |eps|", "tol": 1e-12},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix"],
+                    default="batch")
+    ap.add_argument("--fd-dipoles", type=int, default=4)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=1000)
@@ -358,7 +537,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding this path "
               f"over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
-    run = {"batch": run_batch, "cuboid": run_cuboid, "metasurface": run_metasurface}
+    run = {"batch": run_batch, "cuboid": run_cuboid, "metasurface": run_metasurface,
+           "sphere": run_sphere, "helix": run_helix}
     line = run[args.workload](args)
     print(json.dumps(line), flush=True)
 
