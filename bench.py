@@ -233,9 +233,17 @@ def run_batch(args):
     hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
     kname = (f"k_far_row<float, false, {str(hw).lower()}>" if prec == MIXED
              else "k_far_row<double, false, false>")
+    # HBM bytes per launch of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "far_field_traffic.json")
+    if (prec == MIXED and args.particles == 64 and args.wavelengths == 16
+            and os.path.exists(tf)):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
     roof = {"bound": "valu", "kernel": kname,
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
+            "traffic": traffic, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
             "flops_per_pair": FLOPS_PER_FAR_PAIR,
             "targets_per_item": blocks[1] if prec == MIXED else blocks[0],
             "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0}
@@ -515,12 +523,36 @@ def run_helix(args):
     }
 
 
+def run_copy(args):
+    """A plain device copy of 256 MiB (torch copy_): the reference point that
+    scales the HBM traffic counters (FETCH_SIZE, WRITE_SIZE) of a profile."""
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    nbytes = 2 * 4 * n
+    return {"metric": "device copy GB/s (read + write)", "value": nbytes / dt / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "copy of 256 MiB (64 Mi float32)"},
+            "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix"],
+    ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix",
+                                           "copy"],
                     default="batch")
     ap.add_argument("--fd-dipoles", type=int, default=4)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
@@ -538,7 +570,7 @@ def main():
               f"over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     run = {"batch": run_batch, "cuboid": run_cuboid, "metasurface": run_metasurface,
-           "sphere": run_sphere, "helix": run_helix}
+           "sphere": run_sphere, "helix": run_helix, "copy": run_copy}
     line = run[args.workload](args)
     print(json.dumps(line), flush=True)
 
