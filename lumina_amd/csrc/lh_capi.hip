@@ -164,7 +164,8 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
 // FP32 far field (default: the hardware instructions where their range
 // allows); LUMINA_GRAPH=0 issues every cycle's kernels from the host instead
 // of replaying a captured graph; LUMINA_FAR_BLOCK=64|128|256 fixes the
-// far-field targets per work item (default: timed on the batch);
+// far-field targets per work item, 512 the FP32 one-tensor-per-pair kernel
+// (FP64 then uses 256) (default: timed on the batch);
 // LUMINA_EWALD=pairs evaluates the Ewald tensor per pair even where the
 // dipoles sit on a grid (default: grid tables).
 lh::Tiling tiling_from_env() {
@@ -179,6 +180,10 @@ lh::Tiling tiling_from_env() {
   if (bl) {
     const int b = std::atoi(bl);
     if (b == 64 || b == 128 || b == 256) t.block[0] = t.block[1] = b;
+    if (b == lh::kSymBlock) {
+      t.block[0] = 256;
+      t.block[1] = lh::kSymBlock;
+    }
   }
   return t;
 }

@@ -86,12 +86,16 @@ struct Problem {
 void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,
                    double& eta);
 
-// Far-field launch options. block[] is the autotuner's search space.
+// Far-field launch options. block[] is the autotuner's search space: 64, 128
+// or 256 targets per work item of the row kernels, or kSymBlock for the FP32
+// kernel that evaluates one tensor per unordered pair.
+constexpr int kSymBlock = 512;
 struct Tiling {
   int hw_sincos = 1;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
   int graph = 1;              // replay the restart cycle as a captured HIP graph
-  int block[2] = {0, 0};      // far-field targets per work item [FP64, FP32]:
-                              // 64, 128 or 256; 0 = timed on the batch at first use
+  int block[2] = {0, 0};      // far-field form per precision [FP64, FP32]:
+                              // 64, 128, 256 or kSymBlock (FP32 only);
+                              // 0 = timed on the batch at first use
   int ewald_pairs = 0;        // lattice solves: per-pair evaluation even on a grid
 };
 

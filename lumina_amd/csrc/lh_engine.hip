@@ -22,6 +22,8 @@ void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const
                   const Src64*, double2*, int);
 void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                   const Src32*, double2*, bool, int);
+void launch_far_sym32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                      const Src32*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
                    const double2*, int, double*);
@@ -362,6 +364,8 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     max_rev = std::max(max_rev, p.k * diag / (2.0 * M_PI));
   }
   if (max_rev > 200.0) tiling_.hw_sincos = 0;
+  // the one-tensor-per-pair FP32 kernel uses the hardware sin/cos
+  if (!tiling_.hw_sincos && tiling_.block[1] == kSymBlock) tiling_.block[1] = LH_TILE;
   const int S = (int)probs_.size();
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
 
@@ -526,17 +530,23 @@ void Engine::build_far_items(int p) {
   far_sub_[p].clear();
   far_ewt_[p].clear();
   far_ewp_[p].clear();
-  // before tuning, any block size serves (the far items are rebuilt then)
-  const int tb = tiling_.block[p] > 0 ? tiling_.block[p] : LH_TILE;
+  // before tuning, any block size serves (the far items are rebuilt then);
+  // kSymBlock selects the one-tensor-per-unordered-pair FP32 kernel for the
+  // free-space solves (items T >= S of 256 x 256 tiles), the others keep the
+  // row kernel with 256 targets per item
+  const bool sym = tiling_.block[p] == kSymBlock;
+  const int tb = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   for (size_t s = 0; s < sd_.size(); ++s) {
     const SolveDev& d = sd_[s];
+    const bool free_space = d.lat == 0 && !d.has_sub;
     for (int ti = 0; ti < (d.n + tb - 1) / tb; ++ti)
       for (int tj = 0; tj < d.nt; ++tj) {
         FarItem f{(int)s, ti * tb, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
-        else far_free_[p].push_back(f);
+        else if (!(sym && free_space)) far_free_[p].push_back(f);
+        else if (ti >= tj) far_free_[p].push_back(f);
       }
   }
   drop_graphs();
@@ -563,7 +573,9 @@ void Engine::autotune(int p) {
   LH_HIP(hipEventCreate(&b));
   int best = LH_TILE;
   float best_ms = 0.0f;
-  for (int cand : {64, 128, 256}) {
+  std::vector<int> cands = {64, 128, 256};
+  if (p == 1 && tiling_.hw_sincos) cands.push_back(kSymBlock);
+  for (int cand : cands) {
     tiling_.block[p] = cand;
     build_far_items(p);
     matvec(3, LH_SLOT_E, LH_SLOT_W, 0, p != 0);   // warm-up
@@ -577,7 +589,7 @@ void Engine::autotune(int p) {
       LH_HIP(hipEventElapsedTime(&ms, a, b));
       tot += ms;
     }
-    if (cand == 64 || tot < best_ms) {
+    if (cand == cands.front() || tot < best_ms) {
       best = cand;
       best_ms = tot;
     }
@@ -691,7 +703,8 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   const int nv = (int)vit_.size();
   const int p = fp32 ? 1 : 0;
   const int nfree = (int)far_free_[p].size(), nsub = (int)far_sub_[p].size();
-  const int blk = tiling_.block[p] > 0 ? tiling_.block[p] : LH_TILE;
+  const bool sym = tiling_.block[p] == kSymBlock;
+  const int blk = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   const VecItem* vi = d_vit_.as<VecItem>();
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
@@ -710,8 +723,12 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
       launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                     d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
-                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+    if (sym)
+      launch_far_sym32(st, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
+                       d_src32_.as<Src32>(), d_part_.as<double2>());
+    else
+      launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
+                   d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
     launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
                  d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
     launch_ewt32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),

@@ -314,6 +314,155 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
+// FP32 far field, one tensor per unordered pair (free-space solves).
+//
+// G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
+// evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
+// item is a (target tile T, source tile S) pair with T >= S, 256 threads:
+// wave w holds targets T*256 + 64 w + lane and walks the four 64-source
+// sub-blocks of S. Each lane starts with its own source and the sources, with
+// their transposed accumulators, rotate one lane per step around the wave
+// (DPP wave_ror:1), so after 64 steps every target has met every source of the
+// sub-block and every accumulator is back in its lane. On the diagonal tile
+// (T == S) wave w takes sub-blocks b >= w only, and its own sub-block b == w
+// for steps 1..32 (step 32 one way only), so each unordered pair is counted
+// once. Transposed sums are combined over the four waves in LDS in a fixed
+// order: results are deterministic. Forward sums of T go to partial slice S,
+// transposed sums of S to slice T (on the diagonal both go to slice T).
+// Missing sources of a partial tile carry x = 0 and a copy of a real position:
+// their contribution is exactly 0 (a coincident pair is inside the exclusion
+// radius and skipped).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float wave_ror1(float v) {
+  // DPP wave_ror:1 (0x13C): lane l takes the value of lane l - 1 (lane 0 from 63)
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+}
+
+__global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ items,
+                                                   const SolveDev* __restrict__ solves,
+                                                   const GState* __restrict__ st, int which,
+                                                   const Src32* __restrict__ src,
+                                                   double2* __restrict__ part) {
+  __shared__ float lds_t[4][4][64][6];   // [wave][sub-block][source lane][component]
+  const FarItem it = items[blockIdx.x];
+  if (!takes_part(st, it.s, which)) return;
+  const SolveDev S = solves[it.s];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = it.t0 + 64 * w + lane;
+  const bool tvalid = i < S.n;
+  const Src32* __restrict__ sp = src + S.src_off;
+  const Src32 me = sp[tvalid ? i : 0];
+  const float xi = me.x, yi = me.y, zi = me.z;
+  float vi[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0f;
+  const float k = (float)S.k, inv_k = (float)S.inv_k, c0 = (float)S.c0;
+  const float krev = (float)(S.k * 0.15915494309189535);
+  const float rskip2f = S.rskip2f;
+  const bool diag = it.t0 == it.c0;
+  float hi[6], lo[6], run[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = 0.0f;
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      two_sum(hi[q], lo[q], run[q]);
+      run[q] = 0.0f;
+    }
+  };
+
+  for (int b = 0; b < 4; ++b) {
+    const int jb = it.c0 + 64 * b;
+    float ta[6] = {0, 0, 0, 0, 0, 0};
+    if (jb >= S.n || (diag && b < w)) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) lds_t[w][b][lane][q] = 0.0f;
+      continue;
+    }
+    const int j = jb + lane;
+    const bool sv = j < S.n;
+    const Src32 rs = sp[sv ? j : jb];
+    float xs = rs.x, ys = rs.y, zs = rs.z, vs[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vs[q] = sv ? rs.v[q] : 0.0f;
+
+    // one unordered pair per lane: forward into run, transposed into ta
+    auto step = [&](bool both) {
+      const float dx = xi - xs, dy = yi - ys, dz = zi - zs;
+      const float d2f = r2f_exact(dx, dy, dz);
+      float Ar, Ai, Br, Bi;
+      coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
+      apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+      if (both) apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
+    };
+    auto rotate = [&]() {
+      xs = wave_ror1(xs); ys = wave_ror1(ys); zs = wave_ror1(zs);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        vs[q] = wave_ror1(vs[q]);
+        ta[q] = wave_ror1(ta[q]);
+      }
+    };
+    int home;   // rotation of the accumulators at the end
+    if (diag && b == w) {
+      for (int kk = 1; kk < 32; ++kk) {
+        rotate();
+        step(true);
+      }
+      rotate();
+      step(false);   // pairs (l, l - 32): each lane adds its own direction
+      fold();
+      home = 32;
+    } else {
+      for (int kk = 0; kk < 64; ++kk) {
+        step(true);
+        rotate();
+        if (kk == 31) fold();
+      }
+      fold();
+      home = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lds_t[w][b][(lane - home) & 63][q] = ta[q];
+  }
+  __syncthreads();
+  // thread (w, lane): forward sum of target i; transposed sum of source
+  // c0 + 64 w + lane over the four waves
+  double tr[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    tr[q] = ((double)lds_t[0][w][lane][q] + (double)lds_t[1][w][lane][q]) +
+            ((double)lds_t[2][w][lane][q] + (double)lds_t[3][w][lane][q]);
+  const long long n3 = 3LL * S.n;
+  if (diag) {
+    if (tvalid) {
+      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o[c] = make_double2(((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c],
+                            ((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]);
+    }
+  } else {
+    if (tvalid) {
+      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o[c] = make_double2((double)hi[2 * c] + (double)lo[2 * c],
+                            (double)hi[2 * c + 1] + (double)lo[2 * c + 1]);
+    }
+    const int jj = it.c0 + 64 * w + lane;
+    if (jj < S.n) {
+      double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * jj;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[c] = make_double2(tr[2 * c], tr[2 * c + 1]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reduction: y_i = alpha_i^-1 x_i - sum_tiles part - sum_near T_ij x_j.
 // mode 0 writes y to slot `out_slot`; mode 1 writes r = rhs - y to the
 // residual slot and a per-item partial of |r|^2 to norm_part.
@@ -467,6 +616,13 @@ void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
   else
     hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
+}
+
+void launch_far_sym32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                      const GState* gs, int which, const Src32* src, double2* part) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_far_sym32, dim3(nitems), dim3(256), 0, st, items, solves, gs, which, src,
+                     part);
 }
 
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,

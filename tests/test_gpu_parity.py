@@ -237,15 +237,18 @@ def test_batch_resident_rerun(backend):
 @pytest.mark.parametrize("precision", [FP64, MIXED])
 def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
     """Launch options change scheduling only: the captured-graph cycle and the
-    host-issued cycle, and every far-field block size (the autotuner's
-    choices), give bit-identical moments and iteration counts, because each
-    target's sum over a 256-source tile is formed in the same order."""
+    host-issued cycle, and every row-kernel block size, give bit-identical
+    moments and iteration counts, because each target's sum over a
+    256-source tile is formed in the same order. The FP32 one-tensor-per-pair
+    kernel (block 512) sums in another order: its mixed results agree to the
+    GMRES-IR accuracy."""
     from lumina_amd.solver import Batch
     probs = _batch_probs()
     runs = {}
-    for name, env in [("auto", {}), ("nograph", {"LUMINA_GRAPH": "0"}),
-                      ("b64", {"LUMINA_FAR_BLOCK": "64"}), ("b128", {"LUMINA_FAR_BLOCK": "128"}),
-                      ("b256", {"LUMINA_FAR_BLOCK": "256"})]:
+    opts = [("auto", {}), ("nograph", {"LUMINA_GRAPH": "0", "LUMINA_FAR_BLOCK": "256"}),
+            ("b64", {"LUMINA_FAR_BLOCK": "64"}), ("b128", {"LUMINA_FAR_BLOCK": "128"}),
+            ("b256", {"LUMINA_FAR_BLOCK": "256"}), ("sym", {"LUMINA_FAR_BLOCK": "512"})]
+    for name, env in opts:
         for key in ("LUMINA_GRAPH", "LUMINA_FAR_BLOCK"):
             monkeypatch.delenv(key, raising=False)
         for key, val in env.items():
@@ -263,16 +266,49 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             assert kt["far64_block"] == blk
             if precision == MIXED:
                 assert kt["far32_block"] == blk
+        elif name == "sym":
+            assert kt["far64_block"] == 256
+            if precision == MIXED:
+                assert kt["far32_block"] == 512
         else:
             assert kt["far64_block"] in (64, 128, 256)
-        runs[name] = (cs, [i["iterations"] for i in infos], p)
+        runs[name] = (cs, [i["iterations"] for i in infos], p, kt["far32_block"])
         print(name, runs[name][1], kt["far64_block"], kt["far32_block"], kt["tune_ms"])
-    cs0, it0, p0 = runs["auto"]
-    for name, (cs, it, p) in runs.items():
-        assert it == it0, name
-        assert np.array_equal(cs, cs0), name
-        for a, b_ in zip(p, p0):
-            assert np.array_equal(a, b_), name
+    cs0, it0, p0, _ = runs["b256"]
+    for name, (cs, it, p, f32) in runs.items():
+        exact = not (precision == MIXED and f32 == 512)
+        if exact:
+            assert it == it0, name
+            assert np.array_equal(cs, cs0), name
+            for a, b_ in zip(p, p0):
+                assert np.array_equal(a, b_), name
+        else:
+            for q in range(len(it)):
+                if it0[q] < 200:
+                    assert abs(it[q] - it0[q]) <= 2, (name, q)
+                    assert np.all(np.abs(cs[q] - cs0[q]) <= 1e-9 * np.abs(cs0[q])), (name, q)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 255, 256, 257, 515, 700])
+def test_matvec_fp32_symmetric_kernel(backend, n, monkeypatch):
+    """The FP32 one-tensor-per-unordered-pair kernel against the oracle: ragged
+    tile edges, the diagonal tile, FCD near pairs excluded."""
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", "512")
+    if n == 515:
+        pos, alpha, k, cell = sphere_problem()
+    else:
+        r = np.random.default_rng(100 + n)
+        pos = np.round(r.uniform(-15, 15, size=(n, 3)) / 1.5) * 1.5
+        pos = np.unique(pos, axis=0)
+        alpha = I.isotropic_block(40.0 + 10.0j)
+        k = 2 * np.pi / 480.0
+        cell = 1.5
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    x = rand_x(len(pos), 11)
+    yo = O.Problem(pos, alpha, k, True, cell).matvec(x)
+    e = rel(s.matvec(x, MIXED), yo)
+    print(f"sym fp32 matvec n={len(pos)} rel err {e:.2e}")
+    assert e < 2e-6
 
 
 def test_stored_matrix_session(backend):
