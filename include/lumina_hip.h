@@ -172,16 +172,18 @@ int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, dou
 /* Adjoint gradients of C_ext (no reference counterpart; the backward of the
  * solve, as a torch.autograd.Function would expose it). Solves A p = E (the
  * forward solve of lh_pairsum_compute_cross_sections) and A lam = conj(E)
- * on the session's operator, which is complex symmetric, then
+ * on the session's operator, which is complex symmetric (free space, with or
+ * without the Fresnel image), then
  *   grad_eps[2i], grad_eps[2i+1] = dC_ext/dRe(eps_i), dC_ext/dIm(eps_i),
  *     from dainv_deps = d(alpha_i^-1)/d eps_i ([n][9] or [9] if
  *     dainv_uniform, row-major complex: the caller's material model);
  *   grad_pos[3i+c] = dC_ext/d r_i,c, with the FCD near/far classification
  *     of every pair held fixed; NULL skips it.
  * cross_sections receives C_ext, C_abs, C_sca of the forward solve.
- * LH_UNAVAILABLE: grad_pos with a substrate or lattice, or a lattice with a
- * non-zero Bloch vector (operator not symmetric). LH_CONVERGENCE_FAILURE if
- * either solve hits max_iter (outputs are still written). */
+ * LH_UNAVAILABLE: a lattice (the Ewald operator is not symmetric: pairs at
+ * equal height take sgn 0 = +1 for r and -r in the spectral sum), or grad_pos
+ * with a substrate. LH_CONVERGENCE_FAILURE if either solve hits max_iter
+ * (outputs are still written). */
 int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, size_t max_iter,
                             const lh_c64* dainv_deps, int dainv_uniform,
                             double cross_sections[3], double* grad_eps, double* grad_pos,

@@ -2,9 +2,10 @@
 // derivatives).
 //
 // C_ext = (k / E0^2) Im(E^H p) with A p = E (cda/mod.rs:187-199). The
-// operator is complex symmetric in free space, with the Fresnel image and
-// with a Bloch-periodic lattice at k_par = 0, so the adjoint solve is
-// A lam = conj(E) on the same engine, and for any parameter theta
+// operator is complex symmetric in free space, with or without the Fresnel
+// image (not with a lattice: the Ewald spectral sum takes sgn 0 = +1 for r and
+// -r), so the adjoint solve is A lam = conj(E) on the same engine, and for
+// any parameter theta
 //   dC/dtheta = (k / E0^2) Im(dE^H p + lam^T dE - lam^T dA p).
 //  * eps_i (through alpha_i^-1, caller-supplied D_i = d alpha_i^-1 / d eps_i):
 //    f = -lam_i^T D_i p_i; dC/dRe eps_i = K Im f, dC/dIm eps_i = K Re f.

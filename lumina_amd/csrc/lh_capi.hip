@@ -223,7 +223,6 @@ struct lh_pairsum {
   lh_c64 delta{0.0, 0.0};
   int has_sub = 0;
   int has_lat = 0;
-  int bloch_zero = 1;
 };
 
 struct lh_batch {
@@ -356,8 +355,6 @@ int lh_pairsum_create(lh_backend* b, const lh_pairsum_spec* spec, lh_pairsum** o
     s->has_sub = spec->has_substrate ? 1 : 0;
     s->delta = spec->delta_eps;
     s->has_lat = spec->has_lattice ? 1 : 0;
-    s->bloch_zero = (spec->k_bloch[0] == 0.0 && spec->k_bloch[1] == 0.0 &&
-                     spec->k_bloch[2] == 0.0) ? 1 : 0;
     LH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     std::vector<lh::Geometry> geoms;
     geoms.push_back(make_geometry(spec));
@@ -474,9 +471,11 @@ int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, s
     if (!s || !inc || !dainv_deps || !cross_sections || !grad_eps)
       return fail(LH_INVALID_ARGUMENT, "null argument");
     if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
-    if (s->has_lat && !s->bloch_zero)
+    // with a lattice, G_per(-r) != G_per(r) for pairs at equal height
+    // (sgn 0 = +1 in the spectral sum), so A is not symmetric
+    if (s->has_lat)
       return fail(LH_UNAVAILABLE,
-                  "the adjoint solve needs a complex-symmetric operator (Bloch k = 0)");
+                  "the adjoint solve needs a complex-symmetric operator (no lattice)");
     if (grad_pos && (s->has_sub || s->has_lat))
       return fail(LH_UNAVAILABLE, "position gradients are built for free space only");
     LH_HIP(hipSetDevice(s->b->device));

@@ -173,6 +173,12 @@ def test_adjoint_substrate_and_unavailable(backend):
     geo, _ = P.adjoint_grad(np.linalg.solve(A, e), np.linalg.solve(A, np.conj(e)), D)
     print("substrate eps gradient", relmax(ge, geo))
     assert relmax(ge, geo) < 1e-9
+    # a lattice operator is not symmetric: the adjoint is refused
+    from lumina_amd.solver import LatticeSpec
+    sl = PairSumSession(backend, pos, a, k, True, d,
+                        lattice=LatticeSpec((60.0, 0.0, 0.0), (0.0, 60.0, 0.0)))
+    with pytest.raises(LuminaError):
+        sl.adjoint_grad(IncidentField(), 1e-10, 100, D, positions=False)
 
 
 def test_helix_full_size(backend):

@@ -89,3 +89,23 @@ def test_adjoint_with_substrate_gives_epsilon_gradient_only():
     ge, gr = P.adjoint_grad(p, lam_v, I.inverse_polarisability_derivative(
         I.material("Au").epsilon(lam), d, n_env))
     assert gr is None and np.all(np.isfinite(ge))
+
+
+def test_lattice_operator_is_not_symmetric():
+    """With a 2D lattice the operator is not complex symmetric, even at
+    k_par = 0: for two dipoles at the same height the spectral sum's xz and yz
+    entries take sgn(0) = +1 for r and for -r (ewald.rs:186-273), so
+    G_per(-r) differs from G_per(r) there. The adjoint therefore reuses A only
+    without a lattice."""
+    lam, n_env, d = 600.0, 1.0, 2.0
+    pos = I.discretise(I.Cylinder(4.0, 4.0), d)
+    k = I.wavenumber(lam, n_env)
+    a = I.polarisability_block("Au", lam, d, n_env)
+    lat = dict(a1=(40.0, 0.0, 0.0), a2=(0.0, 40.0, 0.0), eta=0.0, n_real=3, n_recip=3)
+    A = O.Problem(pos, a, k, True, d, lattice=lat).assemble()
+    assert np.max(np.abs(A - A.T)) > 1e-3 * np.max(np.abs(A))
+    # the asymmetric entries are exactly the in-plane pairs' xz / yz entries
+    n = len(pos)
+    D = np.abs(A - A.T).reshape(n, 3, n, 3).transpose(0, 2, 1, 3)   # [i, j, a, b]
+    same_z = np.abs(pos[:, None, 2] - pos[None, :, 2]) < 1e-12
+    assert np.max(D[~same_z]) < 1e-12 * np.max(np.abs(A))
