@@ -126,6 +126,43 @@ def cpu_baseline(probs, budget_s=20.0):
                       f"N={med['n']} problem"}
 
 
+def cpu_matvec_rate(P, n, pairs_per_matvec, rows=None, reps=3):
+    """pair-evals/s of the oracle's on-the-fly matvec on this host: full
+    matvecs (median of reps after a warm-up), or, with rows, the first
+    ``rows`` target rows spread over a thread pool (sampled, scaled by
+    rows / n)."""
+    from concurrent.futures import ThreadPoolExecutor
+    threads = min(16, os.cpu_count() or 1)
+    x = np.random.default_rng(0).standard_normal(3 * n) + 0j
+    if rows is None:
+        P.matvec(x, nthreads=threads)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            P.matvec(x, nthreads=threads)
+            ts.append(time.perf_counter() - t0)
+        dt = float(np.median(ts))
+        return {"value": pairs_per_matvec / dt, "unit": "pair-evals/s", "cores": threads,
+                "kind": "port", "cpu_model": cpu_model(),
+                "sample": f"median of {reps} full oracle matvecs (N={n}) after a warm-up, "
+                          f"{dt:.2f} s each"}
+    # `threads` blocks of rows spread evenly over the dipoles
+    per = max(1, rows // threads)
+    starts = np.linspace(0, n - per, threads).astype(int)
+    chunks = [(int(a), int(a) + per) for a in starts]
+    rows = per * threads
+    def work(c):
+        P.matvec_rows(x, c[0], c[1])
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(work, chunks))
+        dt = time.perf_counter() - t0
+    return {"value": pairs_per_matvec * rows / n / dt, "unit": "pair-evals/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle matvec, {threads} blocks of {per} rows spread over N={n}, on "
+                      f"{threads} threads in {dt:.2f} s, scaled to the full matvec (extrapolated)"}
+
+
 def results_allow_failures(batch):
     """Batch results; solves that hit max_iter are reported like the
     reference's ConvergenceFailure rows (status 5) instead of aborting."""
@@ -297,7 +334,16 @@ def run_cuboid(args):
         log(f"[bench] cuboid {name}: {dt:.2f} s, {info}")
         s.close()
     err = abs(res["mixed"]["cs"][0] - res["fp64"]["cs"][0]) / abs(res["fp64"]["cs"][0])
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as O
+        n = len(pos)
+        cpu = cpu_matvec_rate(O.Problem(pos, alpha, k, True, d), n, pairs(n), rows=4096)
+        mv = res["fp64"]["info"]["matvecs"]
+        cpu["solve_s_extrapolated"] = mv * pairs(n) / cpu["value"]
+        log(f"[bench] cuboid cpu baseline {cpu['value']:.3e} pair-evals/s")
     return {
+        "cpu_baseline": cpu,
         "metric": "Green pair-evals/s (single large mixed-precision CDA solve)",
         "value": res["mixed"]["pair_evals_per_s"], "unit": "pair-evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * res["mixed"]["s"],
@@ -369,7 +415,18 @@ def run_metasurface(args):
          (np.array(res["mixed"]["iterations"]) < args.max_iter)
     # only solves that converged in both precisions compare
     err = float(np.max(np.abs(c32[ok] - c64[ok]) / np.abs(c64[ok]))) if ok.any() else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as O
+        p0 = probs[0]
+        lat = p0["lattice"]
+        P = O.Problem(p0["positions"], p0["polarisability"], p0["k"], True, 2.0,
+                      substrate=(p0["substrate"].z_interface, p0["substrate"].delta_eps),
+                      lattice=dict(a1=lat.a1, a2=lat.a2, eta=0.0, n_real=5, n_recip=5))
+        cpu = cpu_matvec_rate(P, n, 2.0 * n * n, rows=32)
+        log(f"[bench] metasurface cpu baseline {cpu['value']:.3e} pair-evals/s")
     return {
+        "cpu_baseline": cpu,
         "metric": "Green pair-evals/s (periodic metasurface, batched mixed-precision solves)",
         "value": res["mixed"]["pair_evals_per_s"], "unit": "pair-evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * res["mixed"]["s"],
@@ -502,7 +559,12 @@ def run_helix(args):
     log(f"[bench] helix checks: oracle rows {e_or:.2e} ({t_or:.1f} s), FD {fd_err:.2e}, "
         f"mixed vs fp64 gradient {mixed_err:.2e}")
     m = res["mixed"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_matvec_rate(O.Problem(pos, a, k, True, d), n, pairs(n), rows=4096)
+        log(f"[bench] helix cpu baseline {cpu['value']:.3e} pair-evals/s")
     return {
+        "cpu_baseline": cpu,
         "metric": "Green pair-evals/s (helix forward + adjoint solves, mixed precision)",
         "value": m["ev"] / m["s"], "unit": "pair-evals/s", "n_gpus": 1, "steps": 1,
         "warmup": args.warmup, "ms_per_step": 1e3 * m["s"], "higher_is_better": True,
