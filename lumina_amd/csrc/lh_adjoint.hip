@@ -48,7 +48,7 @@ __device__ __forceinline__ cplx ddot(const double d[3], const double* v) {
 // Ap, Bp, B of the point tensor at displacement d (|d|^2 = r2 > 0).
 __device__ __forceinline__ void grad_coeffs(double r2, double k, double inv_k, double c0, cplx& Ap,
                                             cplx& Bp, cplx& B) {
-  const double inv_r = rsqrt(r2);
+  const double inv_r = rsqrt_f64(r2);
   const double r = r2 * inv_r;
   const double kr = k * r;
   double sn, cs;

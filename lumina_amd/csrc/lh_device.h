@@ -36,13 +36,26 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* >= 4 */) {
   return s;
 }
 
+// 1/sqrt(x) for normal x > 0: the hardware estimate refined by one
+// second-order Newton-Raphson step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2,
+// which takes a 2^-26-accurate estimate to full double precision. Zero,
+// infinite and denormal arguments are not handled (pair distances are
+// normal numbers; callers select away r = 0).
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 // Point dyadic Green's tensor G(d) = pf [a I + b d d^T / r^2] with
 // pf = k^2 e^{ikr} / (4 pi r), a = 1 + (ikr - 1)/(kr)^2,
 // b = (3 - 3ikr - (kr)^2)/(kr)^2 (greens.rs:34-71), returned as
-// G = A I + B d d^T. c0 = k^2 / (4 pi).
+// G = A I + B d d^T. c0 = k^2 / (4 pi). skip = true gives A = B = 0 (1/r
+// is taken as 0, r2 must be finite).
 __device__ __forceinline__ void g_coeffs64(double r2, double k, double inv_k, double c0,
-                                          double& Ar, double& Ai, double& Br, double& Bi) {
-  const double inv_r = rsqrt(r2);
+                                          double& Ar, double& Ai, double& Br, double& Bi,
+                                          bool skip = false) {
+  const double inv_r = skip ? 0.0 : rsqrt_f64(r2);
   const double r = r2 * inv_r;
   double s, c;
   sincos_f64(k * r, s, c);

@@ -86,7 +86,7 @@ LH_HD inline void sincos_f64(double x, double& s, double& c) {
   pc = fma(r2, pc, 4.1666666666666664e-02);
   pc = fma(r2, pc, -0.5);
   const double cr = fma(r2, pc, 1.0);
-  const long long qi = (long long)q;
+  const int qi = (int)q;   // |q| < 2^31 for |x| < 1e9
   const double ss = (qi & 1) ? cr : sr;
   const double cc = (qi & 1) ? sr : cr;
   s = (qi & 2) ? -ss : ss;

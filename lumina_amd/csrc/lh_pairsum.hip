@@ -163,12 +163,6 @@ __device__ __forceinline__ float fx(const Src32& r) { return r.x; }
 __device__ __forceinline__ float fy(const Src32& r) { return r.y; }
 __device__ __forceinline__ float fz(const Src32& r) { return r.z; }
 
-// G(d) = A I + B d d^T (greens.rs:34-71), scaled by `scale`.
-__device__ __forceinline__ void coeffs(double r2, double scale, double k, double inv_k, double c0,
-                                       double& Ar, double& Ai, double& Br, double& Bi) {
-  g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi);
-  Ar *= scale; Ai *= scale; Br *= scale; Bi *= scale;
-}
 // FP32: skipped pairs get 1/r = 0 (then A = B = 0; r2 is finite, so
 // r = r2 / r stays finite). HW = true: sin/cos from the hardware
 // v_sin_f32/v_cos_f32 with the phase in revolutions (k / 2 pi = krev);
@@ -289,7 +283,7 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
       } else {
         const bool skip = (d2f <= rskip2f) || (j == i);
         const R r2 = dx * dx + dy * dy + dz * dz;
-        coeffs(skip ? R(1) : r2, skip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+        g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, skip);
       }
       apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, r.v, acc);
       if (SUB) {
@@ -300,7 +294,7 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
         if constexpr (std::is_same<R, float>::value)
           coeffs32<HW>(e2, eskip, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
         else
-          coeffs(eskip ? R(1) : e2, eskip ? R(0) : R(1), k, inv_k, c0, Ar, Ai, Br, Bi);
+          g_coeffs64(e2, k, inv_k, c0, Ar, Ai, Br, Bi, eskip);
         apply<R>(dx, dy, ez, Ar, Ai, Br, Bi, r.w, acc);
       }
     }
