@@ -1,8 +1,9 @@
 """Builds the HIP library liblumina_hip.so in-tree for gfx950.
 
-Every .hip source under csrc/ is compiled with hipcc to an object file and
-the objects are linked into lumina_amd/liblumina_hip.so. Packed FP32 VALU
-instructions are disabled (-packed-fp32-ops) for all device code.
+Every .hip source under csrc/ is compiled with hipcc to an object file, every
+.cpp source (host-only code) with g++ (no FMA contraction), and the objects
+are linked into lumina_amd/liblumina_hip.so. Packed FP32 VALU instructions are
+disabled (-packed-fp32-ops) for all device code.
 """
 from __future__ import annotations
 
@@ -33,8 +34,12 @@ def _hipcc():
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [_hipcc(), *FLAGS, "-c", src, "-o", obj]
+    base, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(OBJ, base + ".o")
+    if ext == ".cpp":
+        cmd = ["g++", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-c", src, "-o", obj]
+    else:
+        cmd = [_hipcc(), *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -43,7 +48,8 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                  if f.endswith(".hip") or f.endswith(".cpp"))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

@@ -193,19 +193,40 @@ __global__ __launch_bounds__(256) void k_ewt_row(const FarItem* __restrict__ ite
   R hi[6], lo[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) hi[q] = lo[q] = R(0);
+  // the table entries of the next source are loaded while the current ones
+  // are applied (the gather latency is the limit of this kernel)
+  R tn[12], tin[12];
+  auto fetch = [&](int j) {
+    const long long e = (long long)(pi - gp[j]);
+    const R* a = td + 12 * e;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) tn[q] = a[q];
+    if (sub) {
+      const R* b = ti + 12 * (e + gz[j]);
+#pragma unroll
+      for (int q = 0; q < 12; ++q) tin[q] = b[q];
+    }
+  };
+  fetch(it.c0);
   for (int j0 = it.c0; j0 < it.c1; j0 += 32) {
     const int j1 = min(j0 + 32, it.c1);
     R acc[6] = {0, 0, 0, 0, 0, 0};
     for (int j = j0; j < j1; ++j) {
-      const long long e = (long long)(pi - gp[j]);
+      R t[12], tt[12];
+#pragma unroll
+      for (int q = 0; q < 12; ++q) {
+        t[q] = tn[q];
+        tt[q] = tin[q];
+      }
+      fetch(min(j + 1, it.c1 - 1));
       R v[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) v[q] = sp[j].v[q];
-      apply_sym<R>(td + 12 * e, v, acc);
+      apply_sym<R>(t, v, acc);
       if (sub) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) v[q] = sp[j].w[q];
-        apply_sym<R>(ti + 12 * (e + gz[j]), v, acc);
+        apply_sym<R>(tt, v, acc);
       }
     }
 #pragma unroll

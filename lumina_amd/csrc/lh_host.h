@@ -1,0 +1,69 @@
+// Host-side geometry preparation of the CDA solve engine: near list,
+// regular-grid detection and Ewald records. Plain C++ (no HIP), so the CPU
+// tests compile and check it with g++ directly.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../include/lumina_hip.h"
+
+namespace lh {
+
+// One dipole geometry (positions + near list), shared by every solve on it.
+struct Geometry {
+  int n = 0;
+  int use_fcd = 0;
+  double cell = 0.0;
+  float rskip2f = 0.0f;        // exclusion radius^2 on FP32 positions (0: no near list)
+  std::vector<double> x, y, z;
+  std::vector<double> cls;     // [n][3] positions for the FCD decision, or empty
+  int kw = 0;                  // ELL width (largest near count of a dipole)
+  std::vector<int> ell_col;    // [kw][n] source index j
+  std::vector<int> ell_tid;    // [kw][n] displacement-table id, -1 = pad (row end)
+  std::vector<double> disp;    // [ntab][4] distinct r_i - r_j and the FCD flag
+  int ntab = 0;
+  long long nnz = 0;
+  // regular grid (lattice solves): x = x0 + ix hx etc. for every dipole
+  int grid = 0;                // 1 when detect_grid found one
+  int gn[3] = {1, 1, 1};       // grid points per axis
+  double g0[3] = {0, 0, 0}, gh[3] = {1, 1, 1};
+  std::vector<int> gpack;      // ix * NYd * NZd + iy * NZd + iz
+  std::vector<int> gizj2;      // 2 iz - (nz - 1)
+};
+
+// Finds a regular grid carrying every dipole (each coordinate within 1e-9
+// relative of x0 + integer * h) and fills the grid fields; leaves grid = 0
+// otherwise.
+void detect_grid(Geometry& g);
+
+// Builds the near list: every pair j != i whose FP32-rounded positions
+// satisfy r2f_exact <= rskip2f, i.e. exactly the pairs the far-field kernels
+// skip. Each pair points to a table of distinct displacements r_i - r_j
+// (FP64), flagged FCD when sqrt(|r_i - r_j|^2) <= 2 cell_size, the
+// reference's own test (greens.rs:102-106), evaluated on g.cls when given.
+void build_near_list(Geometry& g);
+
+struct Problem {
+  int geom = 0;
+  double k = 0.0;
+  std::vector<lh_c64> ainv;     // 9 (uniform) or 9n
+  int ainv_uniform = 1;
+  int has_sub = 0;
+  double z_interface = 0.0;
+  lh_c64 delta{0.0, 0.0};
+  // 2D lattice (LatticeSpec, lattice.rs:29-44) and Bloch vector
+  int has_lat = 0;
+  double a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+  double eta = 0.0;            // <= 0: sqrt(pi / A)
+  int n_real = 5, n_recip = 5;
+  double kb[3] = {0, 0, 0};
+};
+
+// Ewald records of a lattice problem at its k: the real-space images
+// (EwReal, m1 outer, m2 inner over [-n_real, n_real]) followed by the
+// spectral terms (EwRecip, n1 outer, n2 inner, |q| < 1e-10 left out), as
+// doubles; returns the counts and the splitting parameter eta.
+void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,
+                   double& eta);
+
+}  // namespace lh
