@@ -47,12 +47,12 @@ void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int
 
 void launch_ewald_table(hipStream_t, int, const SolveDev*, const int*, const long long*,
                         const double*, double*, float*);
-void launch_ewt64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const Src64*, const int*, const int*, const double*, double2*, int);
-void launch_ewt32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const Src32*, const int*, const int*, const float*, double2*, int);
 void launch_ewp(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                 const Src64*, const double*, double2*, int);
+void launch_ewt_shift32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                        const Src32*, const int*, const int*, const float*, double2*);
+void launch_ewt_shift64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                        const Src64*, const int*, const int*, const double*, double2*);
 
 void launch_near_grad_tensors(hipStream_t, int, const SolveDev*, const int*, const int*,
                               const double*, double*);
@@ -307,9 +307,11 @@ void Engine::build_far_items(int p) {
   for (size_t s = 0; s < sd_.size(); ++s) {
     const SolveDev& d = sd_[s];
     const bool free_space = d.lat == 0 && !d.has_sub;
-    for (int ti = 0; ti < (d.n + tb - 1) / tb; ++ti)
+    // lattice tables: one wave of 64 targets per item (k_ewt_shift)
+    const int tbs = d.lat == 1 ? 64 : tb;
+    for (int ti = 0; ti < (d.n + tbs - 1) / tbs; ++ti)
       for (int tj = 0; tj < d.nt; ++tj) {
-        FarItem f{(int)s, ti * tb, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
+        FarItem f{(int)s, ti * tbs, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
@@ -499,9 +501,9 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
                    d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
     launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
                  d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
-    launch_ewt32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which, d_src32_.as<Src32>(),
-                 d_gpack_.as<int>(), d_gizj2_.as<int>(), d_ewt32_.as<float>(),
-                 d_part_.as<double2>(), blk);
+    launch_ewt_shift32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which,
+                       d_src32_.as<Src32>(), d_gpack_.as<int>(), d_gizj2_.as<int>(),
+                       d_ewt32_.as<float>(), d_part_.as<double2>());
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
@@ -510,9 +512,9 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
-    launch_ewt64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
-                 d_gpack_.as<int>(), d_gizj2_.as<int>(), d_ewt64_.as<double>(),
-                 d_part_.as<double2>(), blk);
+    launch_ewt_shift64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which,
+                       d_src64_.as<Src64>(), d_gpack_.as<int>(), d_gizj2_.as<int>(),
+                       d_ewt64_.as<double>(), d_part_.as<double2>());
   }
   launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
              d_ewl_.as<double>(), d_part_.as<double2>(), blk);

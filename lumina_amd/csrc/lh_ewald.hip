@@ -18,8 +18,8 @@
 //  * grid tables (lat = 1): when the dipoles sit on a regular grid, every
 //    displacement r_i - r_j is a grid offset, and every image displacement
 //    (x_i - x_j, y_i - y_j, z_i + z_j - 2 z0) is one too. G_per is tabulated
-//    once per solve for every offset (k_ewald_table) and the matvec gathers
-//    from the table (k_ewt_row), FP64 or FP32. The table is symmetric in its
+//    once per solve for every offset (k_ewald_table) and the matvec reads
+//    the table (k_ewt_shift), FP64 or FP32. The table is symmetric in its
 //    3x3 indices (both sums are), so 6 complex entries per offset.
 //  * per pair (lat = 2): any other position set evaluates G_per for every
 //    pair inside the matvec (k_ewp_row, FP64).
@@ -164,13 +164,39 @@ __global__ __launch_bounds__(256) void k_ewald_table(const SolveDev* __restrict_
 
 // ---------------------------------------------------------------------------
 // Matvec partials from the tables: targets per lane, sources uniform over the
-// work item (the same tiling and partial slices as the free-space far field).
-// Lanes hold consecutive dipoles, which are consecutive grid points along z,
-// so a wave's gathers for one source read a contiguous stretch of the table.
+// work item (the same source tiles and partial slices as the free-space far
+// field).
 // ---------------------------------------------------------------------------
 
+// Table form for one wave of 64 consecutive targets, using the Toeplitz
+// structure of the tables: when source j is the z-neighbour of source j - 1
+// (pack_j = pack_{j-1} + 1), the direct entry lane l needs is the one lane
+// l - 1 used for j - 1 if target l is the z-neighbour of target l - 1, and
+// the image entry (index + 2 iz_j) is the one lane l + 1 used if target l + 1
+// is the z-neighbour of target l. Those entries move between lanes by DPP
+// shifts; only the lanes at the ends of z-runs (and every lane when the
+// source run breaks) read the tables.
+// (Table indices are linear in the packed coordinates, so the direct shift
+// holds for any pack difference of one; the image shift needs iz_j =
+// iz_{j-1} + 1 as well.)
+// DPP wave_shr:1 (0x138): lane l takes lane l - 1, lane 0 keeps its value;
+// DPP wave_shl:1 (0x130): lane l takes lane l + 1, lane 63 keeps its value.
+template <int CTRL>
+__device__ __forceinline__ float wave_shift(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)h2 << 32) | (long long)(unsigned)l2);
+}
+
 template <typename R, typename SRC>
-__global__ __launch_bounds__(256) void k_ewt_row(const FarItem* __restrict__ items,
+__global__ __launch_bounds__(64) void k_ewt_shift(const FarItem* __restrict__ items,
                                                  const SolveDev* __restrict__ solves,
                                                  const GState* __restrict__ st, int which,
                                                  const SRC* __restrict__ src,
@@ -181,11 +207,17 @@ __global__ __launch_bounds__(256) void k_ewt_row(const FarItem* __restrict__ ite
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
-  const int i = it.t0 + threadIdx.x;
+  const int lane = threadIdx.x;
+  const int i = it.t0 + lane;
   const bool valid = i < S.n;
   const int* __restrict__ gp = gpack + S.pos_off;
   const int* __restrict__ gz = gizj2 + S.pos_off;
-  const int pi = gp[valid ? i : 0] + S.ew_center;
+  const int pi = gp[valid ? i : it.t0] + S.ew_center;
+  // z-neighbours among the targets (invalid lanes never qualify)
+  const int pprev = __builtin_amdgcn_update_dpp(pi, pi, 0x138, 0xF, 0xF, false);
+  const int pnext = __builtin_amdgcn_update_dpp(pi, pi, 0x130, 0xF, 0xF, false);
+  const bool nb_prev = lane > 0 && valid && pi - pprev == 1;
+  const bool nb_next = lane < 63 && (i + 1) < S.n && pnext - pi == 1;
   const SRC* __restrict__ sp = src + S.src_off;
   const R* __restrict__ td = tab + 12 * S.ew_off;
   const R* __restrict__ ti = td + 12 * S.ewn;
@@ -193,49 +225,58 @@ __global__ __launch_bounds__(256) void k_ewt_row(const FarItem* __restrict__ ite
   R hi[6], lo[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) hi[q] = lo[q] = R(0);
-  // the table entries of the next source are loaded while the current ones
-  // are applied (the gather latency is the limit of this kernel)
-  R tn[12], tin[12];
-  auto fetch = [&](int j) {
-    const long long e = (long long)(pi - gp[j]);
-    const R* a = td + 12 * e;
+  R t[12], u[12];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) tn[q] = a[q];
-    if (sub) {
-      const R* b = ti + 12 * (e + gz[j]);
-#pragma unroll
-      for (int q = 0; q < 12; ++q) tin[q] = b[q];
-    }
-  };
-  fetch(it.c0);
+  for (int q = 0; q < 12; ++q) t[q] = u[q] = R(0);
+  int qprev = -2, gprev = -1 << 30;
   for (int j0 = it.c0; j0 < it.c1; j0 += 32) {
     const int j1 = min(j0 + 32, it.c1);
     R acc[6] = {0, 0, 0, 0, 0, 0};
     for (int j = j0; j < j1; ++j) {
-      R t[12], tt[12];
+      const int qj = gp[j], gj = gz[j];
+      // uniform: source j follows j - 1 along the table's fastest index; the
+      // image index moves the other way only if iz also grew by one (always
+      // the case with more than one z plane)
+      const bool run = qj == qprev + 1;
+      const bool run_img = run && gj == gprev + 2;
+      qprev = qj;
+      gprev = gj;
+      const long long e = (long long)(pi - qj);
+      if (run) {
 #pragma unroll
-      for (int q = 0; q < 12; ++q) {
-        t[q] = tn[q];
-        tt[q] = tin[q];
+        for (int q = 0; q < 12; ++q) t[q] = wave_shift<0x138>(t[q]);
       }
-      fetch(min(j + 1, it.c1 - 1));
+      if (!(run && nb_prev)) {
+        const R* a = td + 12 * e;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) t[q] = a[q];
+      }
       R v[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) v[q] = sp[j].v[q];
       apply_sym<R>(t, v, acc);
       if (sub) {
+        if (run_img) {
+#pragma unroll
+          for (int q = 0; q < 12; ++q) u[q] = wave_shift<0x130>(u[q]);
+        }
+        if (!(run_img && nb_next)) {
+          const R* b = ti + 12 * (e + gj);
+#pragma unroll
+          for (int q = 0; q < 12; ++q) u[q] = b[q];
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) v[q] = sp[j].w[q];
-        apply_sym<R>(tt, v, acc);
+        apply_sym<R>(u, v, acc);
       }
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if constexpr (std::is_same<R, float>::value) {
-        const float s = hi[q] + acc[q];
-        const float bb = s - hi[q];
-        lo[q] += (hi[q] - (s - bb)) + (acc[q] - bb);
-        hi[q] = s;
+        const float s2 = hi[q] + acc[q];
+        const float bb = s2 - hi[q];
+        lo[q] += (hi[q] - (s2 - bb)) + (acc[q] - bb);
+        hi[q] = s2;
       } else {
         hi[q] += acc[q];
       }
@@ -300,20 +341,20 @@ void launch_ewald_table(hipStream_t st, int nitems, const SolveDev* solves, cons
                      t64, t32);
 }
 
-void launch_ewt64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
-                  const GState* gs, int which, const Src64* src, const int* gpack,
-                  const int* gizj2, const double* tab, double2* part, int block) {
+void launch_ewt_shift32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                        const GState* gs, int which, const Src32* src, const int* gpack,
+                        const int* gizj2, const float* tab, double2* part) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL((k_ewt_row<double, Src64>), dim3(nitems), dim3(block), 0, st, items,
-                     solves, gs, which, src, gpack, gizj2, tab, part);
+  hipLaunchKernelGGL((k_ewt_shift<float, Src32>), dim3(nitems), dim3(64), 0, st, items, solves,
+                     gs, which, src, gpack, gizj2, tab, part);
 }
 
-void launch_ewt32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
-                  const GState* gs, int which, const Src32* src, const int* gpack,
-                  const int* gizj2, const float* tab, double2* part, int block) {
+void launch_ewt_shift64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                        const GState* gs, int which, const Src64* src, const int* gpack,
+                        const int* gizj2, const double* tab, double2* part) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL((k_ewt_row<float, Src32>), dim3(nitems), dim3(block), 0, st, items,
-                     solves, gs, which, src, gpack, gizj2, tab, part);
+  hipLaunchKernelGGL((k_ewt_shift<double, Src64>), dim3(nitems), dim3(64), 0, st, items, solves,
+                     gs, which, src, gpack, gizj2, tab, part);
 }
 
 void launch_ewp(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,

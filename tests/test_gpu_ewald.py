@@ -200,3 +200,28 @@ def test_ewald_metasurface_rows_full_size(backend):
     e32 = rel(y32, y)
     print("metasurface fp32 table vs fp64", e32)
     assert e32 < 2e-6
+
+
+@pytest.mark.parametrize("shape", ["plane", "two_planes", "ragged"])
+def test_ewald_fp32_table_shapes(backend, shape):
+    """The FP32 table kernel moves entries between lanes along z-runs: check
+    it on a single z plane (no image shift), two planes and a ragged count,
+    against the FP64 table and the oracle."""
+    if shape == "plane":
+        pos = cylinder(12.0, 8.0)
+        pos = pos[np.abs(pos[:, 2] - pos[:, 2].min()) < 1e-9]
+    elif shape == "two_planes":
+        pos = cylinder(12.0, 8.0)
+        zs = np.unique(pos[:, 2])
+        pos = pos[np.isin(pos[:, 2], zs[:2])]
+    else:
+        pos = cylinder(9.0, 10.0)[:301]
+    k, alpha, lat, olat, substrate, osub = setup(pos, period=80.0)
+    s = PairSumSession(backend, pos, alpha, k, True, 2.0, substrate=substrate, lattice=lat)
+    x = rand_x(len(pos), 12)
+    y64 = s.matvec(x, FP64)
+    y32 = s.matvec(x, MIXED)
+    yo = O.Problem(pos, alpha, k, True, 2.0, substrate=osub, lattice=olat).matvec(x)
+    e64, e32 = rel(y64, yo), rel(y32, yo)
+    print(f"ewald table {shape} N={len(pos)}: fp64 {e64:.2e} fp32 {e32:.2e}")
+    assert e64 < 1e-11 and e32 < 2e-6
