@@ -165,7 +165,8 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
 // allows); LUMINA_GRAPH=0 issues every cycle's kernels from the host instead
 // of replaying a captured graph; LUMINA_FAR_BLOCK=64|128|256 fixes the
 // far-field targets per work item, 512 the FP32 one-tensor-per-pair kernel
-// (FP64 then uses 256) (default: timed on the batch);
+// (FP64 then uses 256), sym that kernel in both precisions (default: timed
+// on the batch);
 // LUMINA_EWALD=pairs evaluates the Ewald tensor per pair even where the
 // dipoles sit on a grid (default: grid tables).
 lh::Tiling tiling_from_env() {
@@ -177,6 +178,7 @@ lh::Tiling tiling_from_env() {
   const char* ew = std::getenv("LUMINA_EWALD");
   if (ew && std::string(ew) == "pairs") t.ewald_pairs = 1;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
+  if (bl && std::string(bl) == "sym") t.block[0] = t.block[1] = lh::kSymBlock;
   if (bl) {
     const int b = std::atoi(bl);
     if (b == 64 || b == 128 || b == 256) t.block[0] = t.block[1] = b;

@@ -31,14 +31,14 @@ struct DevMem {
 };
 
 // Far-field launch options. block[] is the autotuner's search space: 64, 128
-// or 256 targets per work item of the row kernels, or kSymBlock for the FP32
-// kernel that evaluates one tensor per unordered pair.
+// or 256 targets per work item of the row kernels, or kSymBlock for the
+// kernels that evaluate one tensor per unordered pair.
 constexpr int kSymBlock = 512;
 struct Tiling {
   int hw_sincos = 1;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
   int graph = 1;              // replay the restart cycle as a captured HIP graph
   int block[2] = {0, 0};      // far-field form per precision [FP64, FP32]:
-                              // 64, 128, 256 or kSymBlock (FP32 only);
+                              // 64, 128, 256 or kSymBlock;
                               // 0 = timed on the batch at first use
   int ewald_pairs = 0;        // lattice solves: per-pair evaluation even on a grid
 };
@@ -132,7 +132,8 @@ class Engine {
   std::vector<FarItem> far_ewt_[2], far_ewp_[2];
   DevMem d_far_ewt_[2], d_far_ewp_[2];
   DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
-  DevMem d_gtab_, d_dainv_, d_gout_;   // adjoint: near gradient tensors, D blocks, output
+  DevMem d_gtab_, d_dainv_, d_gout_, d_far_grad_;   // adjoint: near gradient tensors, D
+                                                     // blocks, output, far items
   long long n_tab_total_ = 0;
   int n_ewitems_ = 0;
   std::vector<int> ewl_cap_;   // records reserved per solve

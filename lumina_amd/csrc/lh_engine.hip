@@ -24,6 +24,8 @@ void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const
                   const Src32*, double2*, bool, int);
 void launch_far_sym32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src32*, double2*);
+void launch_far_sym64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                      const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
                    const double2*, int, double*);
@@ -299,7 +301,7 @@ void Engine::build_far_items(int p) {
   far_ewt_[p].clear();
   far_ewp_[p].clear();
   // before tuning, any block size serves (the far items are rebuilt then);
-  // kSymBlock selects the one-tensor-per-unordered-pair FP32 kernel for the
+  // kSymBlock selects the one-tensor-per-unordered-pair kernel for the
   // free-space solves (items T >= S of 256 x 256 tiles), the others keep the
   // row kernel with 256 targets per item
   const bool sym = tiling_.block[p] == kSymBlock;
@@ -344,7 +346,7 @@ void Engine::autotune(int p) {
   int best = LH_TILE;
   float best_ms = 0.0f;
   std::vector<int> cands = {64, 128, 256};
-  if (p == 1 && tiling_.hw_sincos) cands.push_back(kSymBlock);
+  if (p == 0 || tiling_.hw_sincos) cands.push_back(kSymBlock);
   for (int cand : cands) {
     tiling_.block[p] = cand;
     build_far_items(p);
@@ -508,8 +510,12 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
-                 d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    if (sym)
+      launch_far_sym64(st, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
+                       d_src64_.as<Src64>(), d_part_.as<double2>());
+    else
+      launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
+                   d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_ewt_shift64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which,
@@ -673,9 +679,19 @@ void Engine::gradients(int p_slot, const double dir[3], double amplitude,
                              d_disp_.as<double>(), d_gtab_.as<double>());
     launch_prep_grad(st, nv, d_vit_.as<VecItem>(), sd, d_px_.as<double>(), d_py_.as<double>(),
                      d_pz_.as<double>(), d_vec_.as<double2>(), p_slot, d_src64_.as<Src64>());
-    const int blk = tiling_.block[0] > 0 ? tiling_.block[0] : LH_TILE;
-    launch_grad_far(st, (int)far_free_[0].size(), d_far_free_[0].as<FarItem>(), sd,
-                    d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    // every ordered (256 targets, 256 sources) tile pair of the free-space solves
+    std::vector<FarItem> items;
+    for (size_t s = 0; s < sd_.size(); ++s) {
+      const SolveDev& d = sd_[s];
+      if (d.lat != 0 || d.has_sub) continue;
+      for (int ti = 0; ti < d.nt; ++ti)
+        for (int tj = 0; tj < d.nt; ++tj)
+          items.push_back(FarItem{(int)s, ti * LH_TILE, tj * LH_TILE,
+                                  std::min(d.n, (tj + 1) * LH_TILE)});
+    }
+    upload(d_far_grad_, items, st);
+    launch_grad_far(st, (int)items.size(), d_far_grad_.as<FarItem>(), sd, d_src64_.as<Src64>(),
+                    d_part_.as<double2>(), LH_TILE);
   }
   const size_t obytes = sizeof(double) * 5 * (size_t)std::max<long long>(ntot, 1);
   if (d_gout_.bytes < obytes) d_gout_.alloc(obytes);

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
-// FP32 far field, one tensor per unordered pair (free-space solves).
+// Far field, one tensor per unordered pair (free-space solves), FP32 or FP64.
 //
 // G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
 // evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
@@ -333,67 +333,110 @@ __device__ __forceinline__ float wave_ror1(float v) {
   return __builtin_bit_cast(
       float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
 }
+__device__ __forceinline__ double wave_ror1(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x13C, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
-__global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ items,
-                                                   const SolveDev* __restrict__ solves,
-                                                   const GState* __restrict__ st, int which,
-                                                   const Src32* __restrict__ src,
-                                                   double2* __restrict__ part) {
-  __shared__ float lds_t[4][4][64][6];   // [wave][sub-block][source lane][component]
+// One pair (target lane, source rotated into the lane): the FP32 form uses
+// the FP32 exclusion distance as |d|^2, the FP64 form evaluates |d|^2 in FP64
+// and takes the exclusion decision on the FP32-rounded positions.
+template <typename R> struct SymPos;
+template <> struct SymPos<float> {
+  float x, y, z;
+  __device__ void set(const Src32& r) { x = r.x; y = r.y; z = r.z; }
+  __device__ void rotate() { x = wave_ror1(x); y = wave_ror1(y); z = wave_ror1(z); }
+};
+template <> struct SymPos<double> {
+  double x, y, z;
+  __device__ void set(const Src64& r) { x = r.x; y = r.y; z = r.z; }
+  __device__ void rotate() { x = wave_ror1(x); y = wave_ror1(y); z = wave_ror1(z); }
+};
+
+template <typename R>
+__device__ __forceinline__ void sym_coeffs(const SymPos<R>& t, const SymPos<R>& sp, float rskip2f,
+                                           R k, float krev, R inv_k, R c0, R& dx, R& dy, R& dz,
+                                           R& Ar, R& Ai, R& Br, R& Bi) {
+  dx = t.x - sp.x;
+  dy = t.y - sp.y;
+  dz = t.z - sp.z;
+  if constexpr (std::is_same<R, float>::value) {
+    const float d2f = r2f_exact(dx, dy, dz);
+    coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
+  } else {
+    const float d2f = r2f_exact((float)t.x - (float)sp.x, (float)t.y - (float)sp.y,
+                                (float)t.z - (float)sp.z);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, d2f <= rskip2f);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_far_sym(const FarItem* __restrict__ items,
+                                                 const SolveDev* __restrict__ solves,
+                                                 const GState* __restrict__ st, int which,
+                                                 const typename SrcOf<R>::type* __restrict__ src,
+                                                 double2* __restrict__ part) {
+  typedef typename SrcOf<R>::type SRC;
+  __shared__ R lds_t[4][4][64][6];   // [wave][sub-block][source lane][component]
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = it.t0 + 64 * w + lane;
   const bool tvalid = i < S.n;
-  const Src32* __restrict__ sp = src + S.src_off;
-  const Src32 me = sp[tvalid ? i : 0];
-  const float xi = me.x, yi = me.y, zi = me.z;
-  float vi[6];
+  const SRC* __restrict__ sp = src + S.src_off;
+  const SRC me = sp[tvalid ? i : 0];
+  SymPos<R> tp;
+  tp.set(me);
+  R vi[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0f;
-  const float k = (float)S.k, inv_k = (float)S.inv_k, c0 = (float)S.c0;
+  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : R(0);
+  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0;
   const float krev = (float)(S.k * 0.15915494309189535);
   const float rskip2f = S.rskip2f;
   const bool diag = it.t0 == it.c0;
-  float hi[6], lo[6], run[6];
+  R hi[6], lo[6], run[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = 0.0f;
+  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = R(0);
 
   auto fold = [&]() {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      two_sum(hi[q], lo[q], run[q]);
-      run[q] = 0.0f;
+      if constexpr (std::is_same<R, float>::value) two_sum(hi[q], lo[q], run[q]);
+      else hi[q] += run[q];
+      run[q] = R(0);
     }
   };
 
   for (int b = 0; b < 4; ++b) {
     const int jb = it.c0 + 64 * b;
-    float ta[6] = {0, 0, 0, 0, 0, 0};
+    R ta[6] = {0, 0, 0, 0, 0, 0};
     if (jb >= S.n || (diag && b < w)) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) lds_t[w][b][lane][q] = 0.0f;
+      for (int q = 0; q < 6; ++q) lds_t[w][b][lane][q] = R(0);
       continue;
     }
     const int j = jb + lane;
     const bool sv = j < S.n;
-    const Src32 rs = sp[sv ? j : jb];
-    float xs = rs.x, ys = rs.y, zs = rs.z, vs[6];
+    const SRC rs = sp[sv ? j : jb];
+    SymPos<R> spos;
+    spos.set(rs);
+    R vs[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) vs[q] = sv ? rs.v[q] : 0.0f;
+    for (int q = 0; q < 6; ++q) vs[q] = sv ? rs.v[q] : R(0);
 
     // one unordered pair per lane: forward into run, transposed into ta
     auto step = [&](bool both) {
-      const float dx = xi - xs, dy = yi - ys, dz = zi - zs;
-      const float d2f = r2f_exact(dx, dy, dz);
-      float Ar, Ai, Br, Bi;
-      coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
-      apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
-      if (both) apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
+      R dx, dy, dz, Ar, Ai, Br, Bi;
+      sym_coeffs<R>(tp, spos, rskip2f, k, krev, inv_k, c0, dx, dy, dz, Ar, Ai, Br, Bi);
+      apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+      if (both) apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
     };
     auto rotate = [&]() {
-      xs = wave_ror1(xs); ys = wave_ror1(ys); zs = wave_ror1(zs);
+      spos.rotate();
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         vs[q] = wave_ror1(vs[q]);
@@ -615,8 +658,15 @@ void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
 void launch_far_sym32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
                       const GState* gs, int which, const Src32* src, double2* part) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(k_far_sym32, dim3(nitems), dim3(256), 0, st, items, solves, gs, which, src,
-                     part);
+  hipLaunchKernelGGL((k_far_sym<float>), dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
+                     src, part);
+}
+
+void launch_far_sym64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                      const GState* gs, int which, const Src64* src, double2* part) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL((k_far_sym<double>), dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
+                     src, part);
 }
 
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,

@@ -239,9 +239,9 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
     """Launch options change scheduling only: the captured-graph cycle and the
     host-issued cycle, and every row-kernel block size, give bit-identical
     moments and iteration counts, because each target's sum over a
-    256-source tile is formed in the same order. The FP32 one-tensor-per-pair
-    kernel (block 512) sums in another order: its mixed results agree to the
-    GMRES-IR accuracy."""
+    256-source tile is formed in the same order. The one-tensor-per-pair
+    kernels (block 512) sum in another order: their results agree to the
+    solve tolerance."""
     from lumina_amd.solver import Batch
     probs = _batch_probs()
     runs = {}
@@ -271,12 +271,14 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             if precision == MIXED:
                 assert kt["far32_block"] == 512
         else:
-            assert kt["far64_block"] in (64, 128, 256)
-        runs[name] = (cs, [i["iterations"] for i in infos], p, kt["far32_block"])
+            assert kt["far64_block"] in (64, 128, 256, 512)
+        runs[name] = (cs, [i["iterations"] for i in infos], p,
+                      (kt["far64_block"], kt["far32_block"]))
         print(name, runs[name][1], kt["far64_block"], kt["far32_block"], kt["tune_ms"])
     cs0, it0, p0, _ = runs["b256"]
-    for name, (cs, it, p, f32) in runs.items():
-        exact = not (precision == MIXED and f32 == 512)
+    for name, (cs, it, p, (f64, f32)) in runs.items():
+        # the one-tensor-per-pair kernels (block 512) sum in another order
+        exact = f64 != 512 and not (precision == MIXED and f32 == 512)
         if exact:
             assert it == it0, name
             assert np.array_equal(cs, cs0), name
@@ -287,6 +289,34 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
                 if it0[q] < 200:
                     assert abs(it[q] - it0[q]) <= 2, (name, q)
                     assert np.all(np.abs(cs[q] - cs0[q]) <= 1e-9 * np.abs(cs0[q])), (name, q)
+
+
+@pytest.mark.parametrize("n", [1, 2, 65, 257, 515, 700])
+def test_matvec_fp64_symmetric_kernel(backend, n, monkeypatch):
+    """The FP64 one-tensor-per-unordered-pair kernel against the oracle (FP64
+    bar), and a solve through it against the row kernel's."""
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", "sym")
+    if n == 515:
+        pos, alpha, k, cell = sphere_problem()
+    else:
+        r = np.random.default_rng(200 + n)
+        pos = np.unique(np.round(r.uniform(-15, 15, size=(n, 3)) / 1.5) * 1.5, axis=0)
+        alpha = I.isotropic_block(40.0 + 10.0j)
+        k = 2 * np.pi / 480.0
+        cell = 1.5
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    x = rand_x(len(pos), 13)
+    # a solve first, so that the session's tiling is in force for the matvec
+    out, _, info = s.compute_cross_sections(IncidentField(), 1e-10, 1000)
+    yo = O.Problem(pos, alpha, k, True, cell).matvec(x)
+    e = rel(s.matvec(x, FP64), yo)
+    print(f"sym fp64 matvec n={len(pos)} rel err {e:.2e}")
+    assert e < 1e-12
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", "256")
+    s2 = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
+    out2, _, info2 = s2.compute_cross_sections(IncidentField(), 1e-10, 1000)
+    assert abs(info["iterations"] - info2["iterations"]) <= 1
+    assert np.all(np.abs(out - out2) <= 1e-9 * np.abs(out2))
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 255, 256, 257, 515, 700])
