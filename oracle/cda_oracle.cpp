@@ -1027,6 +1027,64 @@ int or_adjoint_grad(const OrSpec* spec, const double* dir, const double* pol6, d
   return 0;
 }
 
+// fields.rs:18-48 field_at_point at m observation points: the incident
+// field (types.rs:60-74) plus sum_i G(r_obs, r_i) p_i, dipoles with
+// |r_obs - r_i|^2 < 1e-20 skipped; also |E|^2 (fields.rs:87).
+void or_field_at_points(int64_t n, const double* pos, const double* moments, double k,
+                        const double* dir, const double* pol6, double amplitude, int64_t m,
+                        const double* points, double* field, double* intensity) {
+  const C64* P = reinterpret_cast<const C64*>(moments);
+  C64* F = reinterpret_cast<C64*>(field);
+  for (int64_t q = 0; q < m; ++q) {
+    const double* o = points + 3 * q;
+    C64 e[3];
+    or_incident(1, o, dir, pol6, amplitude, k, reinterpret_cast<double*>(e));
+    for (int64_t i = 0; i < n; ++i) {
+      const double* r = pos + 3 * i;
+      double dx = o[0] - r[0], dy = o[1] - r[1], dz = o[2] - r[2];
+      double r_sq = dx * dx + dy * dy + dz * dz;
+      if (r_sq < 1e-20) continue;
+      T33 g;
+      greens_point(o, r, k, g);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) e[a] = add(e[a], mul(g[a][b], P[3 * i + b]));
+    }
+    for (int a = 0; a < 3; ++a) F[3 * q + a] = e[a];
+    intensity[q] = norm_sqr(e[0]) + norm_sqr(e[1]) + norm_sqr(e[2]);
+  }
+}
+
+// fields.rs:156-221 compute_far_field: theta = pi it / max(n_theta - 1, 1),
+// phi = 2 pi ip / n_phi, F = sum_i p_i e^{-ik rhat.r_i}, intensity
+// |(I - rhat rhat) F|^2.
+void or_far_field(int64_t n, const double* pos, const double* moments, double k, int64_t n_theta,
+                  int64_t n_phi, double* theta, double* phi, double* intensity) {
+  const C64* P = reinterpret_cast<const C64*>(moments);
+  int64_t t = 0;
+  for (int64_t it = 0; it < n_theta; ++it) {
+    double th = PI * (double)it / (double)std::max<int64_t>(n_theta - 1, 1);
+    double st = std::sin(th), ct = std::cos(th);
+    for (int64_t ip = 0; ip < n_phi; ++ip, ++t) {
+      double ph = 2.0 * PI * (double)ip / (double)n_phi;
+      double sp = std::sin(ph), cp = std::cos(ph);
+      double rh[3] = {st * cp, st * sp, ct};
+      C64 f[3] = {c(0.0), c(0.0), c(0.0)};
+      for (int64_t i = 0; i < n; ++i) {
+        const double* r = pos + 3 * i;
+        double kdotr = k * (rh[0] * r[0] + rh[1] * r[1] + rh[2] * r[2]);
+        C64 phase = cexp(c(0.0, -kdotr));
+        for (int a = 0; a < 3; ++a) f[a] = add(f[a], mul(P[3 * i + a], phase));
+      }
+      C64 rdotf = add(add(muls(f[0], rh[0]), muls(f[1], rh[1])), muls(f[2], rh[2]));
+      double in = 0.0;
+      for (int a = 0; a < 3; ++a) in += norm_sqr(sub(f[a], muls(rdotf, rh[a])));
+      theta[t] = th;
+      phi[t] = ph;
+      intensity[t] = in;
+    }
+  }
+}
+
 // Point-tensor gradient d G_ab / d (r_o)_c as [c][a][b] complex.
 void or_greens_point_grad(const double* ro, const double* rs, double k, double* out) {
   T333 g;

@@ -54,6 +54,9 @@ def load():
                                          C.POINTER(C.c_int64), C.c_int64]
         _lib.or_adjoint_grad.restype = C.c_int
         _lib.or_greens_point_grad.argtypes = [D, D, C.c_double, D]
+        _lib.or_field_at_points.argtypes = [C.c_int64, D, D, C.c_double, D, D, C.c_double,
+                                            C.c_int64, D, D, D]
+        _lib.or_far_field.argtypes = [C.c_int64, D, D, C.c_double, C.c_int64, C.c_int64, D, D, D]
     return _lib
 
 
@@ -264,6 +267,32 @@ class Problem:
                                     _d(dv.view(np.float64)), uniform, _d(ge), _d(gr),
                                     rp, nrows)
         return ge.reshape(-1, 2), (gr.reshape(-1, 3) if st == 0 else None)
+
+
+def field_at_points(positions, moments, k, points, direction=(0, 0, 1), polarisation=(1, 0, 0),
+                    amplitude=1.0):
+    """fields.rs:18-48 at each point: (E (m, 3) complex, |E|^2 (m,))."""
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
+    mv = np.ascontiguousarray(np.asarray(moments, dtype=np.complex128).reshape(-1))
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    pol = np.array([[complex(q).real, complex(q).imag] for q in polarisation]).reshape(6)
+    f = np.zeros(3 * len(pts), dtype=np.complex128)
+    inten = np.zeros(len(pts))
+    load().or_field_at_points(len(pos), _d(pos), _d(mv.view(np.float64)), float(k),
+                              _d(_v3(direction)), _d(np.ascontiguousarray(pol)), float(amplitude),
+                              len(pts), _d(pts), _d(f.view(np.float64)), _d(inten))
+    return f.reshape(-1, 3), inten
+
+
+def far_field(positions, moments, k, n_theta, n_phi):
+    """fields.rs:156-221: (theta, phi, intensity), each n_theta * n_phi."""
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
+    mv = np.ascontiguousarray(np.asarray(moments, dtype=np.complex128).reshape(-1))
+    m = n_theta * n_phi
+    th, ph, inten = np.zeros(m), np.zeros(m), np.zeros(m)
+    load().or_far_field(len(pos), _d(pos), _d(mv.view(np.float64)), float(k), int(n_theta),
+                        int(n_phi), _d(th), _d(ph), _d(inten))
+    return th, ph, inten
 
 
 def greens_point_grad(ro, rs, k):

@@ -324,3 +324,57 @@ def test_mutation_is_caught():
     rh = d / R
     Gw = pf * (a * np.eye(3) + b_wrong * np.outer(rh, rh))
     assert np.max(np.abs(Gw - G)) / np.max(np.abs(G)) > 1e-6
+
+
+# ---- near and far fields (fields.rs) ---------------------------------------
+
+def test_far_field_single_dipole_closed_form():
+    # one dipole: F = p e^{-ik rhat.r}, intensity |(I - rhat rhat) p|^2
+    p = np.array([1.0 + 0.5j, -0.3j, 0.2])
+    th, ph, inten = O.far_field([[1.0, -2.0, 3.0]], p, 0.02, 7, 8)
+    rh = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    ft = p[None, :] - (rh @ p)[:, None] * rh
+    assert np.allclose(inten, np.sum(np.abs(ft) ** 2, axis=1), rtol=1e-14, atol=1e-15)
+    assert np.allclose(th.reshape(7, 8)[:, 0], np.linspace(0, np.pi, 7))
+    assert np.allclose(ph[:8], 2 * np.pi * np.arange(8) / 8)
+
+
+def test_far_field_two_dipole_interference():
+    # two equal dipoles on the z axis a distance L apart: |1 + e^{-ikL cos th}|^2
+    k, L = 0.015, 120.0
+    p = np.array([0.0, 1.0, 0.0, 0.0, 1.0, 0.0])
+    th, ph, inten = O.far_field([[0, 0, 0], [0, 0, L]], p, k, 9, 4)
+    single = 1.0 - (np.sin(th) * np.sin(ph)) ** 2
+    fac = np.abs(1.0 + np.exp(-1j * k * L * np.cos(th))) ** 2
+    assert np.allclose(inten, single * fac, rtol=1e-12, atol=1e-14)
+
+
+def test_near_field_tends_to_far_field():
+    # at k r ~ 1e4 the scattered near field is k^2 e^{ikr}/(4 pi r) (I - rr) F
+    k = 0.02
+    pos = np.array([[0.0, 0.0, 0.0], [3.0, 1.0, -2.0]])
+    p = np.array([1.0, 0.2j, 0.0, -0.3, 0.5, 0.1j])
+    th0, ph0 = 1.1, 0.7
+    rh = np.array([np.sin(th0) * np.cos(ph0), np.sin(th0) * np.sin(ph0), np.cos(th0)])
+    r = 5e5
+    e, _ = O.field_at_points(pos, p, k, [r * rh], amplitude=0.0)
+    f = sum(p[3 * i:3 * i + 3] * np.exp(-1j * k * rh @ pos[i]) for i in range(2))
+    ft = f - (rh @ f) * rh
+    want = k * k * np.exp(1j * k * r) / (4 * np.pi * r) * ft
+    assert np.max(np.abs(e[0] - want)) < 1e-3 * np.max(np.abs(want))
+
+
+def test_near_field_incident_and_skip():
+    k = 0.02
+    pos = np.array([[0.0, 0.0, 0.0], [2.0, 0.0, 0.0]])
+    p = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    pts = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 7.0]])
+    e, inten = O.field_at_points(pos, np.zeros(6, complex), k, pts, polarisation=(0, 1j, 0),
+                                 amplitude=2.0)
+    # no moments: the incident field E0 e e^{ik d.r}
+    assert np.allclose(e[:, 1], 2.0j * np.exp(1j * k * pts[:, 2]))
+    assert np.allclose(inten, 4.0)
+    # at a dipole's own position that dipole is skipped (fields.rs:35-37)
+    e1, _ = O.field_at_points(pos, p, k, pts[:1], amplitude=0.0)
+    e2, _ = O.field_at_points(pos[1:], p[3:], k, pts[:1], amplitude=0.0)
+    assert np.array_equal(e1, e2)
