@@ -26,7 +26,9 @@ oracle); --workload helix (Au helix, coil 25 nm, pitch 40 nm, 4 turns, wire
 8 nm, d=1.5 nm, 35,013 dipoles, water, 600 nm, left and right circular
 light: forward + adjoint solves and the dC_ext/d(eps_i, r_i) gradients,
 checked against the oracle's CPU adjoint on sampled dipoles and against
-central differences).
+central differences); --workload fields (the solved cuboid's 400x400
+near-field map and 181x360 far-field pattern, FP64, checked against the
+oracle on sampled points).
 """
 from __future__ import annotations
 
@@ -585,6 +587,78 @@ def run_helix(args):
     }
 
 
+def run_fields(args):
+    """Near-field map and far-field pattern of the solved 38,637-dipole
+    cuboid: a 400 x 400 map on the z = 0 plane over +-100 nm (160,000 points,
+    fields.rs:51-108) and a 181 x 360 far-field grid (fields.rs:156-221),
+    both FP64 on the device from the resident solution."""
+    from concurrent.futures import ThreadPoolExecutor
+    from lumina_amd import fields as F
+    from lumina_amd import inputs as I
+    from lumina_amd.solver import MIXED, HipBackend, IncidentField, PairSumSession
+    from oracle import pyoracle as O
+    pos = I.au_cuboid_80x40x40_d1p5()
+    n = len(pos)
+    lam, n_env, d = 520.0, 1.33, 1.5
+    alpha = I.polarisability_block("Au", lam, d, n_env)
+    k = I.wavenumber(lam, n_env)
+    be = HipBackend(0)
+    s = PairSumSession(be, pos, alpha, k, True, d, precision=MIXED)
+    _, p, info = s.compute_cross_sections(IncidentField(), args.tol, args.max_iter)
+    plane = F.NearFieldPlane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 100.0, 100.0, 400, 400)
+    pts, _ = F.plane_points(plane)
+    nt, nph = 181, 360
+    for _ in range(args.warmup):
+        F.field_at_points(s, pts, IncidentField())
+        F.compute_far_field(s, nt, nph)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, inten = F.field_at_points(s, pts, IncidentField())
+    t_near = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ff = F.compute_far_field(s, nt, nph)
+    t_far = (time.perf_counter() - t0) / args.steps
+    near_pairs = float(len(pts)) * n
+    far_pairs = float(nt * nph) * n
+    log(f"[bench] fields: near map {len(pts)} points {1e3 * t_near:.2f} ms "
+        f"({near_pairs / t_near:.3e} pairs/s), far field {nt}x{nph} {1e3 * t_far:.2f} ms "
+        f"({far_pairs / t_far:.3e} dipole-directions/s)")
+    # parity on a sample of map points and directions against the oracle
+    rows = np.random.default_rng(0).choice(len(pts), 64, replace=False)
+    eo, _ = O.field_at_points(pos, p, k, pts[rows])
+    e_near = float(np.max(np.abs(e[rows] - eo)) / np.max(np.abs(eo)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        per = 64
+        chunks = [pts[i * per:(i + 1) * per] for i in range(threads)]
+        with ThreadPoolExecutor(threads) as ex:
+            t1 = time.perf_counter()
+            list(ex.map(lambda c: O.field_at_points(pos, p, k, c), chunks))
+            dt = time.perf_counter() - t1
+        cpu = {"value": per * threads * n / dt, "unit": "pair-evals/s", "cores": threads,
+               "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"oracle field_at_points, {threads} x {per} map points x {n} dipoles on "
+                         f"{threads} threads in {dt:.2f} s"}
+        log(f"[bench] fields cpu baseline {cpu['value']:.3e} pair-evals/s")
+    s.close()
+    return {
+        "cpu_baseline": cpu,
+        "metric": "Green pair-evals/s (near-field map, FP64)",
+        "value": near_pairs / t_near, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_near, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Au cuboid 80x40x40 nm d=1.5 ({n} dipoles), 520 nm, water: "
+                               "400x400 near-field map (z=0, +-100 nm) and 181x360 far field "
+                               "from the mixed-precision solution", "parallelism": "single GPU"},
+        "far_field": {"ms": 1e3 * t_far, "dipole_directions_per_s": far_pairs / t_far,
+                      "directions": nt * nph},
+        "near_rel_err_vs_oracle_64_points": e_near, "solve_info": info,
+        "timing": "wall clock of the C-ABI call: upload of the points, kernels, download",
+    }
+
+
 def run_copy(args):
     """A plain device copy of 256 MiB (torch copy_): the reference point that
     scales the HBM traffic counters (FETCH_SIZE, WRITE_SIZE) of a profile."""
@@ -614,7 +688,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix",
-                                           "copy"],
+                                           "fields", "copy"],
                     default="batch")
     ap.add_argument("--fd-dipoles", type=int, default=4)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
@@ -632,7 +706,7 @@ def main():
               f"over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
     run = {"batch": run_batch, "cuboid": run_cuboid, "metasurface": run_metasurface,
-           "sphere": run_sphere, "helix": run_helix, "copy": run_copy}
+           "sphere": run_sphere, "helix": run_helix, "fields": run_fields, "copy": run_copy}
     line = run[args.workload](args)
     print(json.dumps(line), flush=True)
 

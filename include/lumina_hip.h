@@ -188,6 +188,24 @@ int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, s
                             const lh_c64* dainv_deps, int dainv_uniform,
                             double cross_sections[3], double* grad_eps, double* grad_pos,
                             lh_solve_info* forward_info, lh_solve_info* adjoint_info);
+/* fields.rs:18-48 field_at_point, for m observation points (the loop of
+ * compute_near_field_map, fields.rs:74-91): E = E_inc(obs) + sum_i
+ * G(obs, r_i) p_i with the free-space point tensor (greens.rs:34-71: no
+ * FCD, image or lattice term, as the reference), a dipole with
+ * |obs - r_i|^2 < 1e-20 left out. moments (3N) = the solved moments, or NULL
+ * for the solution of this session's last solve (already on the device).
+ * points: [m][3]; field ([m][3], may be NULL); intensity ([m], may be NULL)
+ * = |E|^2 (fields.rs:87). FP64 throughout. */
+int lh_pairsum_field_at_points(lh_pairsum* s, const lh_c64* moments, const lh_incident* inc,
+                               size_t m, const double* points, lh_c64* field,
+                               double* intensity);
+/* fields.rs:156-221 compute_far_field: theta = pi it / max(n_theta - 1, 1),
+ * phi = 2 pi ip / n_phi (theta-major order), intensity = |(I - rhat rhat) F|^2
+ * with F = sum_i p_i e^{-ik rhat.r_i}. moments as for
+ * lh_pairsum_field_at_points. theta, phi (may be NULL) and intensity:
+ * [n_theta * n_phi]. */
+int lh_pairsum_far_field(lh_pairsum* s, const lh_c64* moments, size_t n_theta, size_t n_phi,
+                         double* theta, double* phi, double* intensity);
 void lh_pairsum_destroy(lh_pairsum* s);
 
 /* ---- batched solves ------------------------------------------------------ */

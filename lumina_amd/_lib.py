@@ -31,7 +31,8 @@ EXPORTS = [
     "lh_session_create", "lh_session_upload_matrix", "lh_session_matvec", "lh_session_destroy",
     "lh_pairsum_create", "lh_pairsum_set_wavelength", "lh_pairsum_matvec", "lh_pairsum_solve",
     "lh_pairsum_incident", "lh_pairsum_cross_sections", "lh_pairsum_compute_cross_sections",
-    "lh_pairsum_adjoint_grad", "lh_pairsum_destroy", "lh_batch_solve", "lh_batch_create", "lh_batch_run",
+    "lh_pairsum_adjoint_grad", "lh_pairsum_field_at_points", "lh_pairsum_far_field",
+    "lh_pairsum_destroy", "lh_batch_solve", "lh_batch_create", "lh_batch_run",
     "lh_batch_results", "lh_batch_destroy", "lh_batch_set_timing", "lh_batch_kernel_times",
 ]
 
@@ -143,6 +144,8 @@ def load():
     lib.lh_pairsum_adjoint_grad.argtypes = [P, C.POINTER(Incident), C.c_double, C.c_size_t, P,
                                             C.c_int, D, D, D, C.POINTER(SolveInfo),
                                             C.POINTER(SolveInfo)]
+    lib.lh_pairsum_field_at_points.argtypes = [P, P, C.POINTER(Incident), C.c_size_t, D, P, D]
+    lib.lh_pairsum_far_field.argtypes = [P, P, C.c_size_t, C.c_size_t, D, D, D]
     lib.lh_pairsum_destroy.argtypes = [P]
     lib.lh_pairsum_destroy.restype = None
     lib.lh_batch_solve.argtypes = [P, C.POINTER(BatchProblem), C.c_size_t, C.c_double,

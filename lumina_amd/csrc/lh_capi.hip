@@ -465,6 +465,60 @@ int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, dou
   });
 }
 
+namespace {
+// Observation points / directions per C-ABI call: kernel indexing is 32-bit.
+constexpr size_t kMaxFieldPoints = (size_t)1 << 28;
+
+int moments_slot(lh_pairsum* s, const lh_c64* moments) {
+  if (!moments) return LH_SLOT_X;
+  s->eng->upload_slot(0, LH_SLOT_W, moments);
+  return LH_SLOT_W;
+}
+}  // namespace
+
+int lh_pairsum_field_at_points(lh_pairsum* s, const lh_c64* moments, const lh_incident* inc,
+                               size_t m, const double* points, lh_c64* field,
+                               double* intensity) {
+  return guarded([&]() {
+    if (!s || !inc || (m > 0 && !points)) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (m > kMaxFieldPoints) return fail(LH_INVALID_ARGUMENT, "too many observation points");
+    if (m == 0 || (!field && !intensity)) return (int)LH_OK;
+    LH_HIP(hipSetDevice(s->b->device));
+    const int slot = moments_slot(s, moments);
+    s->eng->field_at_points(0, slot, *inc, (long long)m, points, field, intensity);
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_far_field(lh_pairsum* s, const lh_c64* moments, size_t n_theta, size_t n_phi,
+                         double* theta, double* phi, double* intensity) {
+  return guarded([&]() {
+    if (!s || !intensity) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (n_phi > 0 && n_theta > kMaxFieldPoints / n_phi)
+      return fail(LH_INVALID_ARGUMENT, "too many directions");
+    const size_t m = n_theta * n_phi;
+    if (m == 0) return (int)LH_OK;
+    std::vector<double> rhat(3 * m);
+    size_t t = 0;
+    for (size_t it = 0; it < n_theta; ++it) {
+      const double th = M_PI * (double)it / (double)std::max<size_t>(n_theta - 1, 1);
+      const double st = std::sin(th), ct = std::cos(th);
+      for (size_t ip = 0; ip < n_phi; ++ip, ++t) {
+        const double ph = 2.0 * M_PI * (double)ip / (double)n_phi;
+        rhat[3 * t] = st * std::cos(ph);
+        rhat[3 * t + 1] = st * std::sin(ph);
+        rhat[3 * t + 2] = ct;
+        if (theta) theta[t] = th;
+        if (phi) phi[t] = ph;
+      }
+    }
+    LH_HIP(hipSetDevice(s->b->device));
+    const int slot = moments_slot(s, moments);
+    s->eng->far_field(0, slot, (long long)m, rhat.data(), intensity);
+    return (int)LH_OK;
+  });
+}
+
 int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, size_t max_iter,
                             const lh_c64* dainv_deps, int dainv_uniform, double cross_sections[3],
                             double* grad_eps, double* grad_pos, lh_solve_info* fwd_info,

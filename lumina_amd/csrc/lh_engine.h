@@ -84,6 +84,14 @@ class Engine {
   void gradients(int p_slot, const double dir[3], double amplitude,
                  const std::vector<lh_c64>& dainv, int dainv_uniform, bool positions,
                  std::vector<double>& out);
+  // fields.rs:18-48: the total field at m observation points (host, 3m
+  // doubles) from the moments in slot `slot` of solve s; field (3m complex,
+  // may be null) and |E|^2 (m, may be null) to the host.
+  void field_at_points(int s, int slot, const lh_incident& inc, long long m, const double* pts,
+                       lh_c64* field, double* intensity);
+  // fields.rs:156-221: |(I - rhat rhat) F|^2 for m directions (host, 3m
+  // doubles) from the moments in slot `slot` of solve s.
+  void far_field(int s, int slot, long long m, const double* rhat, double* intensity);
   hipStream_t stream() const { return stream_; }
 
   // Far-field launch timing (HIP events on the engine's stream).
@@ -133,6 +141,8 @@ class Engine {
   DevMem d_far_ewt_[2], d_far_ewp_[2];
   DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
   DevMem d_gtab_, d_dainv_, d_gout_, d_far_grad_;   // adjoint: near gradient tensors, D
+  DevMem d_fsrc_, d_fpts_, d_fpart_, d_fout_, d_fint_, d_fpol_;   // field maps
+  void field_sources(int s, int slot);
                                                      // blocks, output, far items
   long long n_tab_total_ = 0;
   int n_ewitems_ = 0;

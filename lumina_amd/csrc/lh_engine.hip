@@ -62,6 +62,14 @@ void launch_prep_grad(hipStream_t, int, const VecItem*, const SolveDev*, const d
                       const double*, const double*, const double2*, int, Src64*);
 void launch_grad_far(hipStream_t, int, const FarItem*, const SolveDev*, const Src64*, double2*,
                      int);
+void launch_field_pack(hipStream_t, int, const double*, const double*, const double*,
+                       const double2*, FieldSrc*);
+void launch_field_part(hipStream_t, int, const double*, const FieldSrc*, int, int, double,
+                       double2*);
+void launch_field_reduce(hipStream_t, int, int, const double*, const double2*, double,
+                         const double[3], double, const double*, double2*, double*);
+void launch_ff_part(hipStream_t, int, const double*, const FieldSrc*, int, int, double, double2*);
+void launch_ff_reduce(hipStream_t, int, int, const double*, const double2*, double*);
 void launch_grad_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const int*,
                         const int*, const double*, const double2*, int, const double2*,
                         const double2*, int, long long, const double[3], double, int, double*);
@@ -706,6 +714,97 @@ void Engine::gradients(int p_slot, const double dir[3], double amplitude,
   LH_HIP(hipMemcpyAsync(out.data(), d_gout_.p, sizeof(double) * out.size(),
                         hipMemcpyDeviceToHost, st));
   LH_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// Field maps (lh_fields.hip).
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr long long kFieldBatch = 1LL << 22;   // observation points per launch
+
+// Dipoles per chunk: enough (256-point block, chunk) work groups to fill the
+// chip, at least 64 dipoles per chunk.
+int field_chunk(long long m, int n) {
+  const long long pblocks = (m + 255) / 256;
+  long long nchunk = std::max<long long>(1, (2048 + pblocks - 1) / pblocks);
+  nchunk = std::min<long long>(nchunk, (n + 63) / 64);
+  return (int)((n + nchunk - 1) / nchunk);
+}
+
+void ensure(DevMem& m, size_t bytes) {
+  if (m.p == nullptr || m.bytes < bytes) m.alloc(std::max<size_t>(bytes, 16));
+}
+}  // namespace
+
+void Engine::field_sources(int s, int slot) {
+  const SolveDev& d = sd_[s];
+  ensure(d_fsrc_, sizeof(FieldSrc) * (size_t)d.n);
+  launch_field_pack(stream_, d.n, d_px_.as<double>() + d.pos_off, d_py_.as<double>() + d.pos_off,
+                    d_pz_.as<double>() + d.pos_off,
+                    d_vec_.as<double2>() + d.vec_off + (long long)slot * 3 * d.n,
+                    d_fsrc_.as<FieldSrc>());
+}
+
+void Engine::field_at_points(int s, int slot, const lh_incident& inc, long long m,
+                             const double* pts, lh_c64* field, double* intensity) {
+  const SolveDev& d = sd_[s];
+  hipStream_t st = stream_;
+  field_sources(s, slot);
+  std::vector<double> pol(6);
+  for (int c = 0; c < 3; ++c) {
+    pol[2 * c] = inc.polarisation[c].re;
+    pol[2 * c + 1] = inc.polarisation[c].im;
+  }
+  upload(d_fpol_, pol, st);
+  for (long long q0 = 0; q0 < m; q0 += kFieldBatch) {
+    const int mb = (int)std::min(kFieldBatch, m - q0);
+    const int chunk = field_chunk(mb, d.n);
+    const int nchunk = (d.n + chunk - 1) / chunk;
+    ensure(d_fpts_, sizeof(double) * 3 * (size_t)mb);
+    ensure(d_fpart_, sizeof(double2) * 3 * (size_t)mb * nchunk);
+    ensure(d_fout_, sizeof(double2) * 3 * (size_t)mb);
+    ensure(d_fint_, sizeof(double) * (size_t)mb);
+    LH_HIP(hipMemcpyAsync(d_fpts_.p, pts + 3 * q0, sizeof(double) * 3 * (size_t)mb,
+                          hipMemcpyHostToDevice, st));
+    launch_field_part(st, mb, d_fpts_.as<double>(), d_fsrc_.as<FieldSrc>(), d.n, chunk, d.k,
+                      d_fpart_.as<double2>());
+    launch_field_reduce(st, mb, nchunk, d_fpts_.as<double>(), d_fpart_.as<double2>(), d.k,
+                        inc.direction, inc.amplitude, d_fpol_.as<double>(),
+                        d_fout_.as<double2>(), d_fint_.as<double>());
+    LH_HIP(hipGetLastError());
+    if (field)
+      LH_HIP(hipMemcpyAsync(field + 3 * q0, d_fout_.p, sizeof(double2) * 3 * (size_t)mb,
+                            hipMemcpyDeviceToHost, st));
+    if (intensity)
+      LH_HIP(hipMemcpyAsync(intensity + q0, d_fint_.p, sizeof(double) * (size_t)mb,
+                            hipMemcpyDeviceToHost, st));
+    LH_HIP(hipStreamSynchronize(st));
+  }
+}
+
+void Engine::far_field(int s, int slot, long long m, const double* rhat, double* intensity) {
+  const SolveDev& d = sd_[s];
+  hipStream_t st = stream_;
+  field_sources(s, slot);
+  for (long long q0 = 0; q0 < m; q0 += kFieldBatch) {
+    const int mb = (int)std::min(kFieldBatch, m - q0);
+    const int chunk = field_chunk(mb, d.n);
+    const int nchunk = (d.n + chunk - 1) / chunk;
+    ensure(d_fpts_, sizeof(double) * 3 * (size_t)mb);
+    ensure(d_fpart_, sizeof(double2) * 3 * (size_t)mb * nchunk);
+    ensure(d_fint_, sizeof(double) * (size_t)mb);
+    LH_HIP(hipMemcpyAsync(d_fpts_.p, rhat + 3 * q0, sizeof(double) * 3 * (size_t)mb,
+                          hipMemcpyHostToDevice, st));
+    launch_ff_part(st, mb, d_fpts_.as<double>(), d_fsrc_.as<FieldSrc>(), d.n, chunk, d.k,
+                   d_fpart_.as<double2>());
+    launch_ff_reduce(st, mb, nchunk, d_fpts_.as<double>(), d_fpart_.as<double2>(),
+                     d_fint_.as<double>());
+    LH_HIP(hipGetLastError());
+    LH_HIP(hipMemcpyAsync(intensity + q0, d_fint_.p, sizeof(double) * (size_t)mb,
+                          hipMemcpyDeviceToHost, st));
+    LH_HIP(hipStreamSynchronize(st));
+  }
 }
 
 std::vector<SolveResult> Engine::results() {

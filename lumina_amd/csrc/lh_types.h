@@ -151,6 +151,12 @@ struct Src64 {
   double pad;
 };
 
+// Source record of one solved dipole for the field maps (position, moment).
+struct FieldSrc {
+  double x, y, z, pad;
+  double v[6];          // p_j (re, im) x 3
+};
+
 // FP32 source record.
 struct Src32 {
   typedef float value_type;
