@@ -2,8 +2,9 @@
 
 Every .hip source under csrc/ is compiled with hipcc to an object file, every
 .cpp source (host-only code) with g++ (no FMA contraction), and the objects
-are linked into lumina_amd/liblumina_hip.so. Packed FP32 VALU instructions are
-disabled (-packed-fp32-ops) for all device code.
+are linked into lumina_amd/liblumina_hip.so (with hipFFT, for the
+block-Toeplitz matvec). Packed FP32 VALU instructions are disabled
+(-packed-fp32-ops) for all device code.
 """
 from __future__ import annotations
 
@@ -53,7 +54,9 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(_hipcc())))
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           f"-L{os.path.join(rocm, 'lib')}", "-lhipfft"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

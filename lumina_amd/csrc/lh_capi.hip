@@ -132,8 +132,12 @@ lh::Geometry make_geometry(const lh_pairsum_spec* s) {
   if (g.use_fcd && !s->has_lattice && amax > 1e4 * s->cell_size)
     throw lh::Error(LH_INVALID_ARGUMENT,
                     "positions span more than 1e4 cell sizes from the origin; centre the object");
-  if (s->has_lattice) lh::detect_grid(g);
-  else lh::build_near_list(g);
+  if (s->has_lattice) {
+    lh::detect_grid(g);
+  } else {
+    lh::build_near_list(g);
+    lh::detect_grid(g);   // a full grid qualifies for the FFT matvec
+  }
   return g;
 }
 
@@ -177,6 +181,9 @@ lh::Tiling tiling_from_env() {
   if (gr && std::string(gr) == "0") t.graph = 0;
   const char* ew = std::getenv("LUMINA_EWALD");
   if (ew && std::string(ew) == "pairs") t.ewald_pairs = 1;
+  const char* ff = std::getenv("LUMINA_FFT");
+  if (ff && std::string(ff) == "0") t.fft = 0;
+  if (ff && std::string(ff) == "force") t.fft = 2;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
   if (bl && std::string(bl) == "sym") t.block[0] = t.block[1] = lh::kSymBlock;
   if (bl) {

@@ -87,4 +87,36 @@ __device__ __forceinline__ void g6_point64(double dx, double dy, double dz, doub
   (void)d;
 }
 
+// FCD cell average of the point tensor (greens.rs:120-164): nodes
+// d - (cell/2) n with n in {-sqrt(3/5), 0, sqrt(3/5)}^3, weights
+// w_x w_y w_z from {5/9, 8/9, 5/9}, normalised by the total weight of the
+// nodes kept (a node closer than 1e-10 is left out).
+__device__ __forceinline__ void fcd_average64(double dx, double dy, double dz, double cell,
+                                              double k, double inv_k, double c0, double t[12]) {
+  const double half_d = cell / 2.0;
+  const double nodes[3] = {-sqrt(3.0 / 5.0), 0.0, sqrt(3.0 / 5.0)};
+  const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  double acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) acc[q] = 0.0;
+  double tw = 0.0;
+  for (int ix = 0; ix < 3; ++ix)
+    for (int iy = 0; iy < 3; ++iy)
+      for (int iz = 0; iz < 3; ++iz) {
+        const double ex = dx - half_d * nodes[ix];
+        const double ey = dy - half_d * nodes[iy];
+        const double ez = dz - half_d * nodes[iz];
+        if (r2_exact(ex, ey, ez) < 1e-20) continue;
+        const double w = wts[ix] * wts[iy] * wts[iz];
+        double g[12];
+        g6_point64(ex, ey, ez, k, inv_k, c0, g);
+#pragma unroll
+        for (int q = 0; q < 12; ++q) acc[q] += w * g[q];
+        tw += w;
+      }
+  const double inv_w = (tw > 1e-30) ? 1.0 / tw : 1.0;
+#pragma unroll
+  for (int q = 0; q < 12; ++q) t[q] = acc[q] * inv_w;
+}
+
 }  // namespace lh

@@ -1,6 +1,8 @@
 // Host side of the batched CDA solve engine.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,6 +43,9 @@ struct Tiling {
                               // 64, 128, 256 or kSymBlock;
                               // 0 = timed on the batch at first use
   int ewald_pairs = 0;        // lattice solves: per-pair evaluation even on a grid
+  int fft = 1;                // free-space solves filling a regular grid: FFT block-Toeplitz
+                              // matvec instead of the pair sum; 1 = from kFftMinDipoles
+                              // dipoles, 2 = from 2, 0 = never
 };
 
 struct SolveResult {
@@ -102,6 +107,7 @@ class Engine {
   double last_run_ms = 0.0;
   double tune_ms = 0.0;
   int block(int fp32) const { return tiling_.block[fp32]; }
+  int fft_solves() const { return (int)fft_.size(); }
 
  private:
   struct EvPair {
@@ -141,9 +147,26 @@ class Engine {
   DevMem d_far_ewt_[2], d_far_ewp_[2];
   DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
   DevMem d_gtab_, d_dainv_, d_gout_, d_far_grad_;   // adjoint: near gradient tensors, D
+                                                     // blocks, output, far items
   DevMem d_fsrc_, d_fpts_, d_fpart_, d_fout_, d_fint_, d_fpol_;   // field maps
   void field_sources(int s, int slot);
-                                                     // blocks, output, far items
+  // FFT block-Toeplitz matvec of one full-grid solve (lh_fft.hip)
+  struct FftSolve {
+    int s = 0;
+    int P[3] = {1, 1, 1};      // circulant period per axis
+    int nn[3] = {1, 1, 1};     // grid points per axis
+    double h[3] = {1, 1, 1};   // grid spacing
+    long long len = 0;         // P0 P1 P2
+    hipfftHandle plan64 = nullptr, plan32 = nullptr;   // 3D, batch 3, in place
+    int n_fcd = 0;             // FCD grid offsets folded into K (fft = 2)
+    DevMem K64, K32, X64, X32, pt2dip, dip2pt, fcd_offs;
+    ~FftSolve();
+  };
+  std::vector<std::unique_ptr<FftSolve>> fft_;
+  void setup_fft();                       // plans and index maps
+  void fft_kernels();                     // spectra of the tensor for the current k
+  void fft_matvec(int in_slot, bool fp32);
+  bool has_pair_work() const;             // any solve on the pair-sum kernels
   long long n_tab_total_ = 0;
   int n_ewitems_ = 0;
   std::vector<int> ewl_cap_;   // records reserved per solve

@@ -62,6 +62,17 @@ void launch_prep_grad(hipStream_t, int, const VecItem*, const SolveDev*, const d
                       const double*, const double*, const double2*, int, Src64*);
 void launch_grad_far(hipStream_t, int, const FarItem*, const SolveDev*, const Src64*, double2*,
                      int);
+void launch_fft_kernel(hipStream_t, long long, const int[3], const int[3], const double[3], double,
+                       double2*);
+void launch_fft_to32(hipStream_t, long long, const double2*, float2*);
+void launch_fft_fcd(hipStream_t, int, const int*, long long, const int[3], const double[3], double,
+                    double, double2*);
+void launch_fft_scatter64(hipStream_t, long long, const int*, const double2*, double2*);
+void launch_fft_scatter32(hipStream_t, long long, const int*, const double2*, float2*);
+void launch_fft_mul64(hipStream_t, long long, const double2*, double2*);
+void launch_fft_mul32(hipStream_t, long long, const float2*, float2*);
+void launch_fft_gather64(hipStream_t, int, long long, const int*, const double2*, double, double2*);
+void launch_fft_gather32(hipStream_t, int, long long, const int*, const float2*, double, double2*);
 void launch_field_pack(hipStream_t, int, const double*, const double*, const double*,
                        const double2*, FieldSrc*);
 void launch_field_part(hipStream_t, int, const double*, const FieldSrc*, int, int, double,
@@ -117,6 +128,28 @@ Engine::~Engine() {
   for (auto& g : cycle_graph_)
     if (g) (void)hipGraphExecDestroy(g);
 }
+
+namespace {
+// Smallest 2^a 3^b 5^c 7^d >= m: the circulant period of an axis of n grid
+// points is smooth7(2 n - 1).
+int smooth7(int m) {
+  for (int v = std::max(m, 1);; ++v) {
+    int r = v;
+    for (int f : {2, 3, 5, 7})
+      while (r % f == 0) r /= f;
+    if (r == 1) return v;
+  }
+}
+
+long long fft_len_of(const Geometry& g) {
+  long long len = 1;
+  for (int a = 0; a < 3; ++a) len *= smooth7(2 * g.gn[a] - 1);
+  return len;
+}
+
+// hipFFT batch distances are 32-bit; three component buffers per batch
+constexpr long long kMaxFftLen = 1LL << 28;
+}  // namespace
 
 void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, const Tiling& t) {
   geoms_ = std::move(geoms);
@@ -214,6 +247,16 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     d.dim = p.delta.im;
     d.cell = g.cell;
     d.tol = 0.0;
+    d.fft = 0;
+    std::vector<int> gidx;
+    if (tiling_.fft && !p.has_lat && !p.has_sub &&
+        g.n >= (tiling_.fft == 2 ? 2 : kFftMinDipoles) &&
+        full_grid_indices(g, gidx) && fft_len_of(g) <= kMaxFftLen) {
+      // FCD flags that depend on the grid offset alone go into the FFT
+      // kernel; otherwise the near list keeps the FCD correction
+      std::vector<int> offs;
+      d.fft = fcd_offsets(g, gidx, offs) ? 2 : 1;
+    }
     for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
     for (int u = 0; u < g.ntab; u += 256) {
       tis.push_back(s);
@@ -298,6 +341,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   d_state_.alloc(sizeof(GState) * (size_t)S);
   LH_HIP(hipMemsetAsync(d_state_.p, 0, d_state_.bytes, st));
   d_counter_.alloc(sizeof(int) * 4);
+  setup_fft();
   upload_solves();
   compute_near_tensors();
   LH_HIP(hipStreamSynchronize(st));
@@ -316,6 +360,7 @@ void Engine::build_far_items(int p) {
   const int tb = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   for (size_t s = 0; s < sd_.size(); ++s) {
     const SolveDev& d = sd_[s];
+    if (d.fft) continue;   // G x by FFT
     const bool free_space = d.lat == 0 && !d.has_sub;
     // lattice tables: one wave of 64 targets per item (k_ewt_shift)
     const int tbs = d.lat == 1 ? 64 : tb;
@@ -344,7 +389,17 @@ void Engine::drop_graphs() {
     }
 }
 
+bool Engine::has_pair_work() const {
+  for (const auto& d : sd_)
+    if (!d.fft) return true;
+  return false;
+}
+
 void Engine::autotune(int p) {
+  if (!has_pair_work()) {
+    tiling_.block[p] = LH_TILE;
+    return;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const bool was_timing = timing_;
   timing_ = false;
@@ -431,6 +486,7 @@ void Engine::compute_near_tensors() {
                      d_ewi_e_.as<long long>(), d_ewl_.as<double>(), d_ewt64_.as<double>(),
                      d_ewt32_.as<float>());
   LH_HIP(hipGetLastError());
+  fft_kernels();
 }
 
 void Engine::upload_slot(int s, int slot, const lh_c64* h) {
@@ -468,6 +524,7 @@ void Engine::fill_incident(int s, const lh_incident& inc) {
 }
 
 double Engine::pairs_per_matvec(int s) const {
+  if (sd_[s].fft) return 0.0;   // no pair evaluations in the FFT matvec
   const double n = geoms_[probs_[s].geom].n;
   // with a lattice the self block is a lattice sum too (assembly.rs:241-243)
   const double direct = probs_[s].has_lat ? n * n : n * (n - 1.0);
@@ -532,6 +589,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   }
   launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
              d_ewl_.as<double>(), d_part_.as<double2>(), blk);
+  fft_matvec(in_slot, fp32);
   if (timing_) {
     LH_HIP(hipEventRecord(ev.b, st));
     evs_.push_back(ev);
@@ -714,6 +772,133 @@ void Engine::gradients(int p_slot, const double dir[3], double amplitude,
   LH_HIP(hipMemcpyAsync(out.data(), d_gout_.p, sizeof(double) * out.size(),
                         hipMemcpyDeviceToHost, st));
   LH_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// FFT block-Toeplitz matvec (lh_fft.hip).
+// ---------------------------------------------------------------------------
+
+namespace {
+void check_fft(hipfftResult r, const char* what) {
+  if (r != HIPFFT_SUCCESS)
+    throw Error(LH_DEVICE_ERROR, std::string("hipFFT: ") + what + " failed (" +
+                                     std::to_string((int)r) + ")");
+}
+
+}  // namespace
+
+Engine::FftSolve::~FftSolve() {
+  if (plan64) (void)hipfftDestroy(plan64);
+  if (plan32) (void)hipfftDestroy(plan32);
+}
+
+void Engine::setup_fft() {
+  fft_.clear();
+  hipStream_t st = stream_;
+  for (size_t s = 0; s < sd_.size(); ++s) {
+    if (!sd_[s].fft) continue;
+    const Geometry& g = geoms_[probs_[s].geom];
+    std::vector<int> idx;
+    if (!full_grid_indices(g, idx)) throw Error(LH_DEVICE_ERROR, "FFT grid lost");
+    auto f = std::make_unique<FftSolve>();
+    f->s = (int)s;
+    long long len = 1;
+    for (int a = 0; a < 3; ++a) {
+      f->nn[a] = g.gn[a];
+      f->P[a] = smooth7(2 * g.gn[a] - 1);
+      f->h[a] = g.gh[a];
+      len *= f->P[a];
+    }
+    f->len = len;
+    std::vector<int> pt2dip((size_t)len, -1), dip2pt((size_t)g.n);
+    for (int i = 0; i < g.n; ++i) {
+      const long long q = ((long long)idx[3 * i] * f->P[1] + idx[3 * i + 1]) * f->P[2] +
+                          idx[3 * i + 2];
+      pt2dip[(size_t)q] = i;
+      dip2pt[(size_t)i] = (int)q;
+    }
+    upload(f->pt2dip, pt2dip, st);
+    upload(f->dip2pt, dip2pt, st);
+    if (sd_[s].fft == 2) {
+      std::vector<int> offs;
+      if (!fcd_offsets(g, idx, offs)) throw Error(LH_DEVICE_ERROR, "FFT FCD offsets lost");
+      f->n_fcd = (int)offs.size() / 3;
+      upload(f->fcd_offs, offs, st);
+    }
+    f->K64.alloc(sizeof(double2) * 6 * (size_t)len);
+    f->K32.alloc(sizeof(float2) * 6 * (size_t)len);
+    f->X64.alloc(sizeof(double2) * 3 * (size_t)len);
+    f->X32.alloc(sizeof(float2) * 3 * (size_t)len);
+    LH_HIP(hipMemsetAsync(f->X64.p, 0, f->X64.bytes, st));
+    LH_HIP(hipMemsetAsync(f->X32.p, 0, f->X32.bytes, st));
+    // axes of one point drop out of the transform (same flat layout)
+    int dims[3], rank = 0;
+    for (int a = 0; a < 3; ++a)
+      if (f->P[a] > 1) dims[rank++] = f->P[a];
+    check_fft(hipfftPlanMany(&f->plan64, rank, dims, nullptr, 1, (int)len, nullptr, 1, (int)len,
+                             HIPFFT_Z2Z, 3),
+              "plan Z2Z");
+    check_fft(hipfftPlanMany(&f->plan32, rank, dims, nullptr, 1, (int)len, nullptr, 1, (int)len,
+                             HIPFFT_C2C, 3),
+              "plan C2C");
+    check_fft(hipfftSetStream(f->plan64, st), "set stream");
+    check_fft(hipfftSetStream(f->plan32, st), "set stream");
+    // one transform of each plan outside any graph capture, so that work
+    // buffers and kernels are in place before the first captured cycle
+    check_fft(hipfftExecC2C(f->plan32, f->X32.as<hipfftComplex>(), f->X32.as<hipfftComplex>(),
+                            HIPFFT_FORWARD),
+              "exec C2C");
+    fft_.push_back(std::move(f));
+  }
+  LH_HIP(hipStreamSynchronize(st));
+}
+
+// The tensor at every circulant offset and its spectrum (six components,
+// two batches of three), for the solve's current k; FP32 copy for mixed
+// matvecs.
+void Engine::fft_kernels() {
+  hipStream_t st = stream_;
+  for (auto& fp : fft_) {
+    FftSolve& f = *fp;
+    const SolveDev& d = sd_[f.s];
+    double2* K = f.K64.as<double2>();
+    launch_fft_kernel(st, f.len, f.P, f.nn, f.h, d.k, K);
+    launch_fft_fcd(st, f.n_fcd, f.fcd_offs.as<int>(), f.len, f.P, f.h, d.cell, d.k, K);
+    LH_HIP(hipGetLastError());
+    for (int half = 0; half < 2; ++half) {
+      hipfftDoubleComplex* k3 = reinterpret_cast<hipfftDoubleComplex*>(K + 3 * half * f.len);
+      check_fft(hipfftExecZ2Z(f.plan64, k3, k3, HIPFFT_FORWARD), "exec Z2Z");
+    }
+    launch_fft_to32(st, 6 * f.len, K, f.K32.as<float2>());
+    LH_HIP(hipGetLastError());
+  }
+}
+
+// G x of every FFT solve into its partial slice 0 (k_reduce subtracts it).
+void Engine::fft_matvec(int in_slot, bool fp32) {
+  hipStream_t st = stream_;
+  for (auto& fp : fft_) {
+    FftSolve& f = *fp;
+    const SolveDev& d = sd_[f.s];
+    const double2* x = d_vec_.as<double2>() + d.vec_off + (long long)in_slot * 3 * d.n;
+    double2* part = d_part_.as<double2>() + d.part_off;
+    const double scale = 1.0 / (double)f.len;
+    if (fp32) {
+      hipfftComplex* X = f.X32.as<hipfftComplex>();
+      launch_fft_scatter32(st, f.len, f.pt2dip.as<int>(), x, f.X32.as<float2>());
+      check_fft(hipfftExecC2C(f.plan32, X, X, HIPFFT_FORWARD), "exec C2C");
+      launch_fft_mul32(st, f.len, f.K32.as<float2>(), f.X32.as<float2>());
+      check_fft(hipfftExecC2C(f.plan32, X, X, HIPFFT_BACKWARD), "exec C2C");
+      launch_fft_gather32(st, d.n, f.len, f.dip2pt.as<int>(), f.X32.as<float2>(), scale, part);
+    } else {
+      hipfftDoubleComplex* X = f.X64.as<hipfftDoubleComplex>();
+      launch_fft_scatter64(st, f.len, f.pt2dip.as<int>(), x, f.X64.as<double2>());
+      check_fft(hipfftExecZ2Z(f.plan64, X, X, HIPFFT_FORWARD), "exec Z2Z");
+      launch_fft_mul64(st, f.len, f.K64.as<double2>(), f.X64.as<double2>());
+      check_fft(hipfftExecZ2Z(f.plan64, X, X, HIPFFT_BACKWARD), "exec Z2Z");
+      launch_fft_gather64(st, d.n, f.len, f.dip2pt.as<int>(), f.X64.as<double2>(), scale, part);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -2,6 +2,8 @@
 #include "lh_host.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <unordered_map>
 
@@ -187,6 +189,52 @@ void detect_grid(Geometry& g) {
     g.gh[a] = gh[a];
   }
   g.grid = 1;
+}
+
+bool full_grid_indices(const Geometry& g, std::vector<int>& idx) {
+  idx.clear();
+  if (!g.grid || g.n < 2) return false;
+  const long long cells = (long long)g.gn[0] * g.gn[1] * g.gn[2];
+  if (cells != (long long)g.n) return false;
+  std::vector<char> seen((size_t)cells, 0);
+  idx.resize(3 * (size_t)g.n);
+  const std::vector<double>* ax[3] = {&g.x, &g.y, &g.z};
+  for (int i = 0; i < g.n; ++i) {
+    long long flat = 0;
+    for (int a = 0; a < 3; ++a) {
+      const long long it =
+          g.gn[a] == 1 ? 0 : std::llround(((*ax[a])[i] - g.g0[a]) / g.gh[a]);
+      if (it < 0 || it >= g.gn[a]) return false;
+      idx[3 * (size_t)i + a] = (int)it;
+      flat = flat * g.gn[a] + it;
+    }
+    if (seen[(size_t)flat]) return false;
+    seen[(size_t)flat] = 1;
+  }
+  return true;
+}
+
+bool fcd_offsets(const Geometry& g, const std::vector<int>& idx, std::vector<int>& offs) {
+  offs.clear();
+  std::map<std::array<int, 3>, int> flag;
+  const int n = g.n;
+  for (int kk = 0; kk < g.kw; ++kk)
+    for (int i = 0; i < n; ++i) {
+      const int u = g.ell_tid[(size_t)kk * n + i];
+      if (u < 0) continue;
+      const int j = g.ell_col[(size_t)kk * n + i];
+      const std::array<int, 3> o = {idx[3 * (size_t)i] - idx[3 * (size_t)j],
+                                    idx[3 * (size_t)i + 1] - idx[3 * (size_t)j + 1],
+                                    idx[3 * (size_t)i + 2] - idx[3 * (size_t)j + 2]};
+      const int f = g.disp[4 * (size_t)u + 3] != 0.0 ? 1 : 0;
+      auto it = flag.find(o);
+      if (it == flag.end()) flag.emplace(o, f);
+      else if (it->second != f) return false;
+    }
+  for (const auto& e : flag)
+    if (e.second)
+      for (int c = 0; c < 3; ++c) offs.push_back(e.first[c]);
+  return true;
 }
 
 // lattice.rs:75-147 (cell_measure, eta, reciprocal_vectors), lattice.rs:168-233

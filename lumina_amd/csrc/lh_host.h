@@ -31,10 +31,25 @@ struct Geometry {
   std::vector<int> gizj2;      // 2 iz - (nz - 1)
 };
 
-// Finds a regular grid carrying every dipole (each coordinate within 1e-9
-// relative of x0 + integer * h) and fills the grid fields; leaves grid = 0
-// otherwise.
+// Finds a regular grid carrying every dipole (each coordinate within 1e-12
+// of x0 + integer * h, relative to the coordinate span's magnitude) and
+// fills the grid fields; leaves grid = 0 otherwise.
 void detect_grid(Geometry& g);
+
+// Free-space solves with at least this many dipoles on a full grid use the
+// FFT block-Toeplitz matvec.
+constexpr int kFftMinDipoles = 4096;
+
+// For a geometry whose grid (detect_grid) has every node occupied exactly
+// once (fft_matvec.rs:20-75: a full box of gn[0] x gn[1] x gn[2] points),
+// the grid index (ix, iy, iz) of every dipole, [n][3]; false otherwise.
+bool full_grid_indices(const Geometry& g, std::vector<int>& idx);
+
+// On a full grid (idx from full_grid_indices): true when every near pair at
+// one grid offset (ix_i - ix_j, iy_i - iy_j, iz_i - iz_j) carries the same
+// FCD flag, so that the pair tensor is a function of the offset alone; offs
+// then lists the FCD offsets, [m][3], in increasing lexicographic order.
+bool fcd_offsets(const Geometry& g, const std::vector<int>& idx, std::vector<int>& offs);
 
 // Builds the near list: every pair j != i whose FP32-rounded positions
 // satisfy r2f_exact <= rskip2f, i.e. exactly the pairs the far-field kernels

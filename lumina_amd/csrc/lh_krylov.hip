@@ -293,30 +293,36 @@ __global__ __launch_bounds__(64) void k_zero(int* __restrict__ counter) {
 }
 
 // hp_i = sum over items of the dot partials; hc accumulates over passes.
-__global__ __launch_bounds__(64) void k_fin_dots(const SolveDev* __restrict__ solves,
-                                                GState* __restrict__ st, int j,
-                                                const double2* __restrict__ dot_part,
-                                                int pass) {
+// Four waves per solve, wave w takes i = w, w + 4, ...; its lanes stride
+// over the items and combine with a fixed butterfly (deterministic).
+__global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ solves,
+                                                 GState* __restrict__ st, int j,
+                                                 const double2* __restrict__ dot_part,
+                                                 int pass) {
   const int s = blockIdx.x;
   GState& g = st[s];
   if (!g.inner) return;
   const SolveDev S = solves[s];
-  const int i = threadIdx.x;
-  if (i <= j) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = w; i <= j; i += 4) {
     double sr = 0.0, si = 0.0;
-    for (int b = 0; b < S.nvi; ++b) {
+    for (int b = lane; b < S.nvi; b += 64) {
       const double2 d = dot_part[(long long)(S.vi_off + b) * LH_NDOT + i];
       sr += d.x;
       si += d.y;
     }
-    g.hp_re[i] = sr;
-    g.hp_im[i] = si;
-    if (pass == 0) {
-      g.hc_re[i] = sr;
-      g.hc_im[i] = si;
-    } else {
-      g.hc_re[i] += sr;
-      g.hc_im[i] += si;
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) {
+      g.hp_re[i] = sr;
+      g.hp_im[i] = si;
+      if (pass == 0) {
+        g.hc_re[i] = sr;
+        g.hc_im[i] = si;
+      } else {
+        g.hc_re[i] += sr;
+        g.hc_im[i] += si;
+      }
     }
   }
 }
@@ -334,33 +340,41 @@ __global__ __launch_bounds__(64) void k_fin_step(const SolveDev* __restrict__ so
     return;
   }
   const SolveDev S = solves[s];
+  // the new column, the rotations so far: one parallel load into LDS, then
+  // the serial Givens sweep of thread 0 reads them from there and carries
+  // the rotated entry in a register
+  __shared__ double2 col[LH_MMAX + 1], sns[LH_MMAX];
+  __shared__ double css[LH_MMAX];
+  const int tid = threadIdx.x;
+  if (tid <= j) col[tid] = make_double2(g.hc_re[tid], g.hc_im[tid]);
+  if (tid < j) {
+    css[tid] = g.cs[tid];
+    sns[tid] = make_double2(g.sn_re[tid], g.sn_im[tid]);
+  }
   const double nsq = sum_parts(norm_part + S.vi_off, S.nvi);
-  if (threadIdx.x != 0) return;
+  __syncthreads();
+  if (tid != 0) return;
   const int M = LH_MMAX;
   const double hn = sqrt(nsq);
-  for (int i = 0; i <= j; ++i) {
-    g.H_re[i * M + j] = g.hc_re[i];
-    g.H_im[i * M + j] = g.hc_im[i];
-  }
-  g.H_re[(j + 1) * M + j] = hn;
-  g.H_im[(j + 1) * M + j] = 0.0;
   g.nscale = (hn > 1e-30) ? 1.0 / hn : 0.0;  // lucky breakdown pushes a zero vector
-  auto H = [&](int r, int c) { return make_double2(g.H_re[r * M + c], g.H_im[r * M + c]); };
   auto setH = [&](int r, int c, double2 v) {
     g.H_re[r * M + c] = v.x;
     g.H_im[r * M + c] = v.y;
   };
+  // rotation i acts on rows i, i+1 of the column; row i+1 comes out of it
+  // as the next rotation's row i
+  double2 hij = col[0];
   for (int i = 0; i < j; ++i) {
-    const double c = g.cs[i];
-    const double2 sn = make_double2(g.sn_re[i], g.sn_im[i]);
-    const double2 hij = H(i, j), hi1 = H(i + 1, j);
+    const double c = css[i];
+    const double2 sn = sns[i];
+    const double2 hi1 = col[i + 1];
     const double2 temp = cadd(make_double2(c * hij.x, c * hij.y), cmul(sn, hi1));
     const double2 nsn = make_double2(-sn.x, sn.y);  // -conj(sn)
-    setH(i + 1, j, cadd(cmul(nsn, hij), make_double2(c * hi1.x, c * hi1.y)));
+    hij = cadd(cmul(nsn, hij), make_double2(c * hi1.x, c * hi1.y));
     setH(i, j, temp);
   }
   // givens_rotation (iterative.rs:180-191)
-  const double2 a = H(j, j), b = H(j + 1, j);
+  const double2 a = hij, b = make_double2(hn, 0.0);
   double c;
   double2 sn;
   if (cabs_(b) < 1e-30) {
@@ -523,7 +537,7 @@ void launch_zero(hipStream_t st, int* counter) {
 }
 void launch_fin_dots(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
                      const double2* dp, int pass) {
-  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(64), 0, st, solves, gs, j, dp, pass);
+  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, j, dp, pass);
 }
 void launch_fin_step(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
                      const double* np, int fp32, int* counter) {

@@ -47,39 +47,21 @@ __global__ __launch_bounds__(256) void k_near_tensors(const SolveDev* __restrict
   if (b >= nitems) return;
   const SolveDev S = solves[item_s[b]];
   const int u = item_u[b] + threadIdx.x;
-  if (u >= S.ntab) return;
+  if (u >= S.ntab || S.fft == 2) return;   // fft == 2: FCD inside the FFT kernel
   const double* d = disp + 4 * (S.dtab_off + u);
   const double dx = d[0], dy = d[1], dz = d[2];
   double t[12];
-  if (d[3] != 0.0) {
-    // greens.rs:120-164: nodes r_j + (d/2) n, n in {-sqrt(3/5), 0, sqrt(3/5)}^3,
-    // weights w_x w_y w_z from {5/9, 8/9, 5/9}, normalised by the total weight.
-    const double half_d = S.cell / 2.0;
-    const double nodes[3] = {-sqrt(3.0 / 5.0), 0.0, sqrt(3.0 / 5.0)};
-    const double wts[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
-    double acc[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-    double tw = 0.0;
-    for (int ix = 0; ix < 3; ++ix)
-      for (int iy = 0; iy < 3; ++iy)
-        for (int iz = 0; iz < 3; ++iz) {
-          const double ex = dx - half_d * nodes[ix];
-          const double ey = dy - half_d * nodes[iy];
-          const double ez = dz - half_d * nodes[iz];
-          if (r2_exact(ex, ey, ez) < 1e-20) continue;
-          const double w = wts[ix] * wts[iy] * wts[iz];
-          double g[12];
-          g6_point64(ex, ey, ez, S.k, S.inv_k, S.c0, g);
-#pragma unroll
-          for (int q = 0; q < 12; ++q) acc[q] += w * g[q];
-          tw += w;
-        }
-    const double inv_w = (tw > 1e-30) ? 1.0 / tw : 1.0;
-#pragma unroll
-    for (int q = 0; q < 12; ++q) t[q] = acc[q] * inv_w;
-  } else {
+  if (d[3] != 0.0)
+    fcd_average64(dx, dy, dz, S.cell, S.k, S.inv_k, S.c0, t);
+  else
     g6_point64(dx, dy, dz, S.k, S.inv_k, S.c0, t);
+  if (S.fft) {
+    // the FFT matvec applies the point tensor to every pair: the near list
+    // carries the FCD correction only (exactly zero for point pairs)
+    double g[12];
+    g6_point64(dx, dy, dz, S.k, S.inv_k, S.c0, g);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) t[q] -= g[q];
   }
   double* o = tab + 12 * (S.tab_off + u);
 #pragma unroll
@@ -541,7 +523,8 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
       yr[a] = sr;
       yi[a] = si;
     }
-    for (int ch = 0; ch < S.nt; ++ch) {
+    const int nslice = S.fft ? 1 : S.nt;   // the FFT matvec fills slice 0
+    for (int ch = 0; ch < nslice; ++ch) {
       const double2* p = part + S.part_off + (long long)ch * n3 + 3LL * i;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -550,7 +533,8 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
         yi[c] -= q.y;
       }
     }
-    for (int kk = 0; kk < S.kw; ++kk) {
+    const int kw = S.fft == 2 ? 0 : S.kw;   // fft == 2: no near list
+    for (int kk = 0; kk < kw; ++kk) {
       const long long e = S.ell_off + (long long)kk * S.n + i;
       const int u = ell_tid[e];
       if (u < 0) break;  // pads sit at the end of a row

@@ -77,7 +77,7 @@ struct SolveDev {
   long long ewl_off;   // first EwReal record (EwRecip records follow)
   int ew_nyd, ew_nzd;  // table extents along y and z (2 n - 1)
   int ew_center;       // table index of the zero offset
-  int pad2;
+  int fft;             // 1: G x by the FFT block-Toeplitz matvec (one partial slice)
   double ew_eta;
   double ew_h[3];      // grid spacing
   double ew_zimg0;     // 2 z_min - 2 z_interface (image z at iz_i + iz_j = 0)

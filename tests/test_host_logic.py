@@ -23,7 +23,7 @@ using namespace lh;
 int main() {
   int mode;
   if (std::scanf("%d", &mode) != 1) return 1;
-  if (mode == 0 || mode == 1) {
+  if (mode == 0 || mode == 1 || mode == 3) {
     // near list (0) or grid (1): n fcd cell rskip2f has_cls, then positions, then cls
     Geometry g;
     int n, fcd, has_cls;
@@ -36,7 +36,16 @@ int main() {
       g.cls.resize(3 * n);
       for (int i = 0; i < 3 * n; ++i) std::scanf("%lf", &g.cls[i]);
     }
-    if (mode == 0) {
+    if (mode == 3) {
+      build_near_list(g);
+      detect_grid(g);
+      std::vector<int> idx, offs;
+      const bool full = full_grid_indices(g, idx);
+      const bool ok = full && fcd_offsets(g, idx, offs);
+      std::printf("%d %d %d\n", full ? 1 : 0, ok ? 1 : 0, (int)offs.size() / 3);
+      for (size_t q = 0; q < offs.size(); q += 3)
+        std::printf("%d %d %d\n", offs[q], offs[q + 1], offs[q + 2]);
+    } else if (mode == 0) {
       build_near_list(g);
       std::printf("%d %lld %d\n", g.kw, g.nnz, g.ntab);
       for (int kk = 0; kk < g.kw; ++kk)
@@ -49,10 +58,14 @@ int main() {
                     g.disp[4 * u + 3]);
     } else {
       detect_grid(g);
-      std::printf("%d %d %d %d %.17g %.17g %.17g\n", g.grid, g.gn[0], g.gn[1], g.gn[2], g.gh[0],
-                  g.gh[1], g.gh[2]);
+      std::vector<int> idx;
+      const bool full = full_grid_indices(g, idx);
+      std::printf("%d %d %d %d %.17g %.17g %.17g %d\n", g.grid, g.gn[0], g.gn[1], g.gn[2],
+                  g.gh[0], g.gh[1], g.gh[2], full ? 1 : 0);
       if (g.grid)
-        for (int i = 0; i < n; ++i) std::printf("%d %d\n", g.gpack[i], g.gizj2[i]);
+        for (int i = 0; i < n; ++i)
+          std::printf("%d %d %d %d %d\n", g.gpack[i], g.gizj2[i], full ? idx[3 * i] : -1,
+                      full ? idx[3 * i + 1] : -1, full ? idx[3 * i + 2] : -1);
     }
   } else {
     // Ewald records: a1 a2 eta nr nq kb k
@@ -169,13 +182,15 @@ def test_near_list_classification_positions(harness):
         assert np.array_equal(disp[u, :3], moved[i] - moved[j])
 
 
-def grid(exe, pos):
+def grid(exe, pos, full=False):
     lines = [f"1 {len(pos)} 0 1.0 0.0 0"] + [" ".join(repr(float(v)) for v in p) for p in pos]
     out = run(exe, "\n".join(lines) + "\n")
     head = out[0].split()
     ok, gn, gh = int(head[0]), list(map(int, head[1:4])), list(map(float, head[4:7]))
     packs = np.array([list(map(int, l.split())) for l in out[1:1 + len(pos)]]) if ok else None
-    return ok, gn, gh, packs
+    if full:
+        return int(head[7]), packs[:, 2:] if ok else None
+    return ok, gn, gh, packs[:, :2] if ok else None
 
 
 def test_grid_detection(harness):
@@ -236,3 +251,66 @@ def test_ewald_records_match_lattice_helpers(harness):
     want = np.array(want)
     assert nq == len(want)
     assert np.allclose(rec[nr:], want, rtol=1e-13, atol=1e-18)
+
+
+def test_full_grid_indices(harness):
+    """A box with every node occupied once (fft_matvec.rs:20-75, the
+    reference's make_grid and its displaced-dipole case) gives each dipole's
+    (ix, iy, iz); a missing, displaced or doubled node does not."""
+    n = 4
+    pos = np.array([[i * 1.0, j * 1.0, k * 1.0] for i in range(n) for j in range(n)
+                    for k in range(n)])
+    full, idx = grid(harness, pos, full=True)
+    assert full == 1
+    assert np.array_equal(idx, np.rint(pos).astype(int))
+    perm = np.random.default_rng(1).permutation(len(pos))
+    aniso = pos[perm] * np.array([1.5, 2.0, 0.5]) + np.array([-3.0, 7.0, 0.25])
+    full, idx = grid(harness, aniso, full=True)
+    assert full == 1 and np.array_equal(idx, np.rint(pos[perm]).astype(int))
+    moved = pos.copy()
+    moved[7, 0] += 0.1
+    assert grid(harness, moved, full=True)[0] == 0
+    assert grid(harness, pos[1:], full=True)[0] == 0
+    doubled = pos.copy()
+    doubled[5] = doubled[6]
+    assert grid(harness, doubled, full=True)[0] == 0
+    # one plane and one line are full grids too
+    assert grid(harness, pos[pos[:, 2] == 0.0], full=True)[0] == 1
+    assert grid(harness, pos[(pos[:, 1] == 0.0) & (pos[:, 2] == 0.0)], full=True)[0] == 1
+
+
+def fcd_offsets(exe, pos, cell, cls=None):
+    r = 2.0 * cell * 1.05
+    rskip = np.float32(r * r)
+    lines = [f"3 {len(pos)} 1 {cell!r} {float(rskip)!r} {0 if cls is None else 1}"]
+    lines += [" ".join(repr(float(v)) for v in p) for p in pos]
+    if cls is not None:
+        lines += [" ".join(repr(float(v)) for v in p) for p in cls]
+    out = run(exe, "\n".join(lines) + "\n")
+    full, ok, m = map(int, out[0].split())
+    offs = np.array([list(map(int, l.split())) for l in out[1:1 + m]]).reshape(-1, 3)
+    return full, ok, offs
+
+
+def test_fcd_offsets_of_a_full_grid(harness):
+    """The FCD flag as a function of the grid offset: on a 6 x 5 x 4 box at
+    the cell spacing every offset with |o| h <= 2 cell is FCD (the reference's
+    dist <= 2 cell_size, greens.rs:102-106); a classification position that
+    moves one pair across 2 cell makes the flags offset-dependent."""
+    h = 1.5
+    pos = np.array([[i * h, j * h, k * h] for i in range(6) for j in range(5)
+                    for k in range(4)])
+    full, ok, offs = fcd_offsets(harness, pos, h)
+    assert full == 1 and ok == 1
+    want = sorted((a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)
+                  if (a, b, c) != (0, 0, 0) and a * a + b * b + c * c <= 4)
+    assert [tuple(o) for o in offs] == want
+    # a cell of 1.2 h: offsets up to |o| h <= 2.4 h, i.e. |o|^2 <= 5.76
+    full, ok, offs = fcd_offsets(harness, pos, 1.2 * h)
+    assert ok == 1 and len(offs) == sum(1 for a in range(-2, 3) for b in range(-2, 3)
+                                        for c in range(-2, 3)
+                                        if (a, b, c) != (0, 0, 0) and a * a + b * b + c * c <= 5)
+    cls = pos.copy()
+    cls[37, 0] -= 1e-9        # (ix = 1): the pair with ix = 3 now sits just beyond 2 cell
+    full, ok, _ = fcd_offsets(harness, pos, h, cls=cls)
+    assert full == 1 and ok == 0
