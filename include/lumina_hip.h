@@ -230,6 +230,61 @@ int lh_batch_solve(lh_backend* backend, const lh_batch_problem* problems, size_t
                    double tol, size_t max_iter, int precision, double* cross_sections,
                    lh_c64* const* moments, lh_solve_info* infos);
 
+/* ---- wavelength sweep with the per-wavelength set-up on the device ------ */
+
+/* A tabulated permittivity eps(lambda), interpolated by natural cubic
+ * splines of Re eps and Im eps through the points (spline.rs:32-90) and
+ * defined on [wavelength_nm[0], wavelength_nm[n_points - 1]]
+ * (johnson_christy.rs:266-279; a Johnson-Christy table enters as
+ * eps = (n + i k)^2 at its wavelengths). */
+typedef struct {
+  size_t n_points;               /* >= 2 */
+  const double* wavelength_nm;   /* strictly increasing */
+  const lh_c64* epsilon;
+} lh_material;
+
+/* The per-dipole optical model of the CLI run (runner.rs:150-186). The
+ * geometry part of `geometry` is used as for lh_pairsum_create (positions,
+ * use_fcd, cell_size, lattice, k_bloch, has_substrate and z_interface,
+ * precision, classify_positions); its polarisability, k and delta_eps are
+ * ignored: they are set up on the device for every wavelength. Dipole i has
+ * material material[i] (NULL: all 0), volume spacing[i]^3 (NULL:
+ * uniform_spacing^3) and, when hint[i] >= 0, the ellipsoid tensor of
+ * types.rs:286-317 with depolarisation factors hint_depol[3 hint[i] ..] and
+ * rotation hint_rotation[9 hint[i] ..] (row-major); otherwise the
+ * Clausius-Mossotti polarisability with the radiative correction
+ * (types.rs:243-259). The substrate reflection factor (has_substrate) comes
+ * from material substrate_material, retarded or quasi-static
+ * (substrate.rs:148-197). */
+typedef struct {
+  lh_pairsum_spec geometry;
+  const int32_t* material;
+  const double* spacing;
+  double uniform_spacing;
+  const int32_t* hint;
+  size_t n_hints;
+  const double* hint_depol;
+  const double* hint_rotation;
+  double n_env;                  /* k = 2 pi n_env / lambda, eps_m = n_env^2 */
+  int substrate_material;
+  int substrate_retarded;
+} lh_sweep_spec;
+
+/* The CLI wavelength sweep (runner.rs:145-218) as one batched solve: the
+ * geometry and its near list are built once, then for every wavelength the
+ * permittivities (splines), polarisabilities and their inverses, the
+ * substrate factor and the incident field are computed by device kernels
+ * straight into the batch, and all wavelengths are solved together.
+ * cross_sections: [n_wavelengths][3] (C_ext, C_abs, C_sca); a wavelength
+ * that does not converge gets a NaN row (runner.rs:189-203) and the call
+ * returns LH_CONVERGENCE_FAILURE after writing every row. infos may be NULL.
+ * A wavelength outside a used material's table is LH_INVALID_ARGUMENT
+ * (MaterialError::OutOfRange). */
+int lh_sweep_solve(lh_backend* backend, const lh_sweep_spec* spec, const lh_material* materials,
+                   size_t n_materials, const double* wavelengths_nm, size_t n_wavelengths,
+                   const lh_incident* incident, double tol, size_t max_iter,
+                   double* cross_sections, lh_solve_info* infos);
+
 /* The same, split so that a batch stays resident on the device and can be
  * solved repeatedly: create uploads geometry, alpha^-1, near tensors and
  * incident fields; run solves every problem; results reads them back. */

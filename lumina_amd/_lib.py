@@ -34,6 +34,7 @@ EXPORTS = [
     "lh_pairsum_adjoint_grad", "lh_pairsum_field_at_points", "lh_pairsum_far_field",
     "lh_pairsum_destroy", "lh_batch_solve", "lh_batch_create", "lh_batch_run",
     "lh_batch_results", "lh_batch_destroy", "lh_batch_set_timing", "lh_batch_kernel_times",
+    "lh_sweep_solve",
 ]
 
 
@@ -85,6 +86,20 @@ class SolveInfo(C.Structure):
 
 class BatchProblem(C.Structure):
     _fields_ = [("spec", PairsumSpec), ("incident", Incident)]
+
+
+class Material(C.Structure):
+    _fields_ = [("n_points", C.c_size_t), ("wavelength_nm", C.POINTER(C.c_double)),
+                ("epsilon", C.c_void_p)]
+
+
+class SweepSpec(C.Structure):
+    _fields_ = [("geometry", PairsumSpec), ("material", C.POINTER(C.c_int32)),
+                ("spacing", C.POINTER(C.c_double)), ("uniform_spacing", C.c_double),
+                ("hint", C.POINTER(C.c_int32)), ("n_hints", C.c_size_t),
+                ("hint_depol", C.POINTER(C.c_double)), ("hint_rotation", C.POINTER(C.c_double)),
+                ("n_env", C.c_double), ("substrate_material", C.c_int),
+                ("substrate_retarded", C.c_int)]
 
 
 class KernelTimes(C.Structure):
@@ -158,6 +173,9 @@ def load():
     lib.lh_batch_destroy.restype = None
     lib.lh_batch_set_timing.argtypes = [P, C.c_int]
     lib.lh_batch_kernel_times.argtypes = [P, C.POINTER(KernelTimes)]
+    lib.lh_sweep_solve.argtypes = [P, C.POINTER(SweepSpec), C.POINTER(Material), C.c_size_t, D,
+                                   C.c_size_t, C.POINTER(Incident), C.c_double, C.c_size_t, D,
+                                   C.POINTER(SolveInfo)]
     _lib = lib
     return lib
 

@@ -5,8 +5,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -172,7 +174,9 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
 // (FP64 then uses 256), sym that kernel in both precisions (default: timed
 // on the batch);
 // LUMINA_EWALD=pairs evaluates the Ewald tensor per pair even where the
-// dipoles sit on a grid (default: grid tables).
+// dipoles sit on a grid (default: grid tables); LUMINA_FFT=0 keeps the pair
+// sum for full-grid solves, LUMINA_FFT=force takes the FFT matvec for any
+// full grid of two or more dipoles (default: from 4,096 dipoles).
 lh::Tiling tiling_from_env() {
   lh::Tiling t;
   const char* sc = std::getenv("LUMINA_FP32_SINCOS");
@@ -722,6 +726,154 @@ int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
     out->tune_ms = bt->eng->tune_ms;
     return (int)LH_OK;
   });
+}
+
+int lh_sweep_solve(lh_backend* b, const lh_sweep_spec* spec, const lh_material* materials,
+                   size_t n_materials, const double* wavelengths_nm, size_t n_wl,
+                   const lh_incident* inc, double tol, size_t max_iter, double* cross_sections,
+                   lh_solve_info* infos) {
+  lh_batch* bt = nullptr;
+  int rc = guarded([&]() {
+    if (!b || !spec || !materials || n_materials == 0 || !wavelengths_nm || n_wl == 0 || !inc ||
+        !cross_sections)
+      return fail(LH_INVALID_ARGUMENT, "bad argument");
+    if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
+    if (!(spec->n_env > 0.0) || !std::isfinite(spec->n_env))
+      return fail(LH_INVALID_ARGUMENT, "n_env must be positive and finite");
+    // the geometry checks of a session, with placeholder optics
+    lh_pairsum_spec g = spec->geometry;
+    const lh_c64 unit[9] = {{1, 0}, {0, 0}, {0, 0}, {0, 0}, {1, 0}, {0, 0}, {0, 0}, {0, 0}, {1, 0}};
+    g.polarisability = unit;
+    g.polarisability_uniform = 1;
+    g.k = 1.0;
+    g.delta_eps = lh_c64{0.0, 0.0};
+    check_spec(&g);
+    const size_t n = g.n_dipoles;
+    lh::SweepSetup su;
+    su.n_env = spec->n_env;
+    // materials: knots and spline second derivatives (spline.rs:32-66)
+    for (size_t m = 0; m < n_materials; ++m) {
+      const lh_material& mt = materials[m];
+      if (mt.n_points < 2 || !mt.wavelength_nm || !mt.epsilon)
+        return fail(LH_INVALID_ARGUMENT, "material " + std::to_string(m) + ": need >= 2 points");
+      const int np = (int)mt.n_points;
+      std::vector<double> re(np), im(np), y2(np);
+      for (int q = 0; q < np; ++q) {
+        if (!std::isfinite(mt.wavelength_nm[q]) || !std::isfinite(mt.epsilon[q].re) ||
+            !std::isfinite(mt.epsilon[q].im) || (q > 0 && !(mt.wavelength_nm[q] > mt.wavelength_nm[q - 1])))
+          return fail(LH_INVALID_ARGUMENT,
+                      "material " + std::to_string(m) + ": wavelengths must increase strictly");
+        re[q] = mt.epsilon[q].re;
+        im[q] = mt.epsilon[q].im;
+      }
+      su.mat_off.push_back((int)su.xs.size());
+      su.mat_n.push_back(np);
+      su.xs.insert(su.xs.end(), mt.wavelength_nm, mt.wavelength_nm + np);
+      su.yr.insert(su.yr.end(), re.begin(), re.end());
+      su.yi.insert(su.yi.end(), im.begin(), im.end());
+      lh::natural_spline(mt.wavelength_nm, re.data(), np, y2.data());
+      su.y2r.insert(su.y2r.end(), y2.begin(), y2.end());
+      lh::natural_spline(mt.wavelength_nm, im.data(), np, y2.data());
+      su.y2i.insert(su.y2i.end(), y2.begin(), y2.end());
+    }
+    // optical classes: (material, volume, hint), volume = spacing^3 as powi(3)
+    std::vector<char> used(n_materials, 0);
+    std::map<std::tuple<int, uint64_t, int>, int> classes;
+    std::vector<int> cls_of(n);
+    for (size_t i = 0; i < n; ++i) {
+      const int m = spec->material ? spec->material[i] : 0;
+      if (m < 0 || (size_t)m >= n_materials)
+        return fail(LH_INVALID_ARGUMENT, "material index out of range at dipole " + std::to_string(i));
+      const double d = spec->spacing ? spec->spacing[i] : spec->uniform_spacing;
+      if (!(d > 0.0) || !std::isfinite(d))
+        return fail(LH_INVALID_ARGUMENT, "spacing must be positive at dipole " + std::to_string(i));
+      const int h = spec->hint ? spec->hint[i] : -1;
+      if (h >= 0 && ((size_t)h >= spec->n_hints || !spec->hint_depol || !spec->hint_rotation))
+        return fail(LH_INVALID_ARGUMENT, "hint index out of range at dipole " + std::to_string(i));
+      const double vol = d * d * d;
+      const auto key = std::make_tuple(m, __builtin_bit_cast(uint64_t, vol), h < 0 ? -1 : h);
+      auto it = classes.find(key);
+      if (it == classes.end()) {
+        it = classes.emplace(key, (int)su.cls_material.size()).first;
+        su.cls_material.push_back(m);
+        su.cls_hint.push_back(h < 0 ? -1 : h);
+        su.cls_volume.push_back(vol);
+      }
+      cls_of[i] = it->second;
+      used[m] = 1;
+    }
+    if (su.cls_material.size() > 1) su.cls_of = cls_of;
+    if (spec->hint && spec->n_hints > 0) {
+      su.hint_depol.assign(spec->hint_depol, spec->hint_depol + 3 * spec->n_hints);
+      su.hint_rot.assign(spec->hint_rotation, spec->hint_rotation + 9 * spec->n_hints);
+    }
+    if (g.has_substrate) {
+      const int m = spec->substrate_material;
+      if (m < 0 || (size_t)m >= n_materials)
+        return fail(LH_INVALID_ARGUMENT, "substrate material index out of range");
+      su.sub_material = m;
+      su.sub_retarded = spec->substrate_retarded ? 1 : 0;
+      used[m] = 1;
+    }
+    // MaterialError::OutOfRange (johnson_christy.rs:266-274)
+    for (size_t w = 0; w < n_wl; ++w) {
+      const double L = wavelengths_nm[w];
+      if (!(L > 0.0) || !std::isfinite(L))
+        return fail(LH_INVALID_ARGUMENT, "wavelengths must be positive and finite");
+      for (size_t m = 0; m < n_materials; ++m) {
+        if (!used[m]) continue;
+        const double lo = materials[m].wavelength_nm[0];
+        const double hi = materials[m].wavelength_nm[materials[m].n_points - 1];
+        if (L < lo || L > hi)
+          return fail(LH_INVALID_ARGUMENT, "material " + std::to_string(m) + ": wavelength " +
+                                               std::to_string(L) + " nm outside [" +
+                                               std::to_string(lo) + ", " + std::to_string(hi) + "]");
+      }
+      su.lam.push_back(L);
+    }
+    // one geometry, one problem per wavelength; optics filled on the device
+    LH_HIP(hipSetDevice(b->device));
+    std::vector<lh::Geometry> geoms;
+    geoms.push_back(make_geometry(&g));
+    std::vector<lh::Problem> probs;
+    const bool uniform = su.cls_material.size() == 1;
+    for (size_t w = 0; w < n_wl; ++w) {
+      lh::Problem p = make_problem(&g, 0);
+      p.k = 2.0 * M_PI * spec->n_env / wavelengths_nm[w];   // cda/mod.rs:169-171
+      p.ainv_uniform = uniform ? 1 : 0;
+      p.ainv.assign(uniform ? 9 : 9 * n, lh_c64{0.0, 0.0});
+      p.delta = lh_c64{0.0, 0.0};
+      probs.push_back(std::move(p));
+    }
+    auto owned = std::make_unique<lh_batch>();
+    owned->b = b;
+    owned->amp.assign(n_wl, inc->amplitude);
+    LH_HIP(hipStreamCreateWithFlags(&owned->stream, hipStreamNonBlocking));
+    bt = owned.release();
+    bt->eng = std::make_unique<lh::Engine>(bt->stream);
+    bt->eng->build(geoms, probs, tiling_from_env());
+    bt->eng->sweep_setup(su, *inc);
+    bt->eng->solve(tol, (long long)max_iter, g.precision == LH_PREC_MIXED, true);
+    const auto res = bt->eng->results();
+    const auto cs = bt->eng->cross_sections(bt->amp);
+    int failed = 0;
+    for (size_t w = 0; w < n_wl; ++w) {
+      const bool ok = res[w].status == LH_OK;
+      failed += ok ? 0 : 1;
+      for (int c = 0; c < 3; ++c)
+        cross_sections[3 * w + c] = ok ? cs[3 * w + c] : std::nan("");
+      if (infos) fill_info(res[w], &infos[w]);
+    }
+    if (failed)
+      return fail(LH_CONVERGENCE_FAILURE, std::to_string(failed) + " of " + std::to_string(n_wl) +
+                                              " wavelengths did not converge within " +
+                                              std::to_string(max_iter) + " iterations");
+    return (int)LH_OK;
+  });
+  std::string msg = g_last_error;
+  lh_batch_destroy(bt);
+  g_last_error = msg;
+  return rc;
 }
 
 int lh_batch_solve(lh_backend* b, const lh_batch_problem* problems, size_t count, double tol,

@@ -48,6 +48,21 @@ struct Tiling {
                               // dipoles, 2 = from 2, 0 = never
 };
 
+// Device arrays of a sweep's per-wavelength setup (lh_sweep.hip).
+struct SweepDevArgs {
+  int S, C;
+  double n_env;
+  const double* lam;
+  const int *cmat, *chint, *cls_of;
+  const double *cvol, *hdepol, *hrot;
+  const int *moff, *mcnt;
+  const double *xs, *yr, *yi, *y2r, *y2i;
+  int sub_material, sub_retarded;
+  double2* cls_ainv;
+  double2* delta;
+  int* bad;
+};
+
 struct SolveResult {
   int status = 0;
   int iterations = 0, cycles = 0, matvecs = 0, matvecs32 = 0;
@@ -97,6 +112,10 @@ class Engine {
   // fields.rs:156-221: |(I - rhat rhat) F|^2 for m directions (host, 3m
   // doubles) from the moments in slot `slot` of solve s.
   void far_field(int s, int slot, long long m, const double* rhat, double* intensity);
+  // The per-wavelength setup of a sweep on the device (lh_sweep.hip): every
+  // solve's alpha^-1 blocks, substrate reflection factor and incident field
+  // (RHS and E slots). Throws LH_INVALID_ARGUMENT on a singular block.
+  void sweep_setup(const SweepSetup& su, const lh_incident& inc);
   hipStream_t stream() const { return stream_; }
 
   // Far-field launch timing (HIP events on the engine's stream).

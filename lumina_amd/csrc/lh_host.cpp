@@ -237,6 +237,20 @@ bool fcd_offsets(const Geometry& g, const std::vector<int>& idx, std::vector<int
   return true;
 }
 
+void natural_spline(const double* xs, const double* ys, int n, double* y2) {
+  std::vector<double> u((size_t)std::max(n - 1, 1), 0.0);
+  for (int i = 0; i < n; ++i) y2[i] = 0.0;
+  for (int i = 1; i < n - 1; ++i) {
+    const double sig = (xs[i] - xs[i - 1]) / (xs[i + 1] - xs[i - 1]);
+    const double p = sig * y2[i - 1] + 2.0;
+    y2[i] = (sig - 1.0) / p;
+    const double ui = (ys[i + 1] - ys[i]) / (xs[i + 1] - xs[i]) -
+                      (ys[i] - ys[i - 1]) / (xs[i] - xs[i - 1]);
+    u[i] = (6.0 * ui / (xs[i + 1] - xs[i - 1]) - sig * u[i - 1]) / p;
+  }
+  for (int k = n - 3; k >= 0; --k) y2[k + 1] = y2[k + 1] * y2[k + 2] + u[k + 1];
+}
+
 // lattice.rs:75-147 (cell_measure, eta, reciprocal_vectors), lattice.rs:168-233
 // (point order) and the r-independent factors of ewald.rs:119-273.
 void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,

@@ -58,6 +58,30 @@ bool fcd_offsets(const Geometry& g, const std::vector<int>& idx, std::vector<int
 // reference's own test (greens.rs:102-106), evaluated on g.cls when given.
 void build_near_list(Geometry& g);
 
+// Second derivatives y2 of the natural cubic spline through (xs, ys), n >= 2
+// points with strictly increasing xs (spline.rs:32-66: the tridiagonal
+// sweep of Numerical Recipes with y2[0] = y2[n-1] = 0).
+void natural_spline(const double* xs, const double* ys, int n, double* y2);
+
+// What the device needs to set up every wavelength of a sweep
+// (runner.rs:150-186): the optical classes of the dipoles (one class = one
+// material, volume and depolarisation hint), the permittivity splines and
+// the substrate material.
+struct SweepSetup {
+  std::vector<double> lam;          // [S] wavelength of each solve, nm
+  double n_env = 1.0;
+  std::vector<int> cls_of;          // [n] class of each dipole; empty: one class
+  std::vector<int> cls_material;    // [C] material of the class
+  std::vector<int> cls_hint;        // [C] ellipsoid hint, -1: isotropic
+  std::vector<double> cls_volume;   // [C] spacing^3
+  std::vector<double> hint_depol;   // [H][3] depolarisation factors
+  std::vector<double> hint_rot;     // [H][9] rotation, row-major
+  std::vector<int> mat_off, mat_n;  // knots of material m: [mat_off[m], mat_off[m] + mat_n[m])
+  std::vector<double> xs, yr, yi, y2r, y2i;   // knots, Re/Im eps and their spline y2
+  int sub_material = -1;            // substrate material, -1: none
+  int sub_retarded = 1;             // retarded (1) or quasi-static (0) reflection factor
+};
+
 struct Problem {
   int geom = 0;
   double k = 0.0;

@@ -67,6 +67,14 @@ int main() {
           std::printf("%d %d %d %d %d\n", g.gpack[i], g.gizj2[i], full ? idx[3 * i] : -1,
                       full ? idx[3 * i + 1] : -1, full ? idx[3 * i + 2] : -1);
     }
+  } else if (mode == 4) {
+    // natural spline second derivatives: n, then n (x, y) pairs
+    int n;
+    std::scanf("%d", &n);
+    std::vector<double> xs(n), ys(n), y2(n);
+    for (int i = 0; i < n; ++i) std::scanf("%lf %lf", &xs[i], &ys[i]);
+    natural_spline(xs.data(), ys.data(), n, y2.data());
+    for (int i = 0; i < n; ++i) std::printf("%.17g\n", y2[i]);
   } else {
     // Ewald records: a1 a2 eta nr nq kb k
     Problem p;
@@ -314,3 +322,18 @@ def test_fcd_offsets_of_a_full_grid(harness):
     cls[37, 0] -= 1e-9        # (ix = 1): the pair with ix = 3 now sits just beyond 2 cell
     full, ok, _ = fcd_offsets(harness, pos, h, cls=cls)
     assert full == 1 and ok == 0
+
+
+@pytest.mark.parametrize("name", ["Au", "Ag"])
+def test_natural_spline_matches_restatement(harness, name):
+    """The spline second derivatives of the sweep set-up (spline.rs:32-66)
+    against the restated NaturalSpline of the host inputs, on both
+    Johnson-Christy tables (Re and Im eps), and the two-point case (a
+    straight line: y2 = 0)."""
+    m = I.material(name)
+    for spl in (m.re, m.im):
+        text = f"4 {len(spl.xs)}\n" + "\n".join(f"{x!r} {y!r}" for x, y in zip(spl.xs, spl.ys))
+        y2 = np.array([float(v) for v in run(harness, text + "\n")[:len(spl.xs)]])
+        assert np.array_equal(y2, np.array(spl.y2))
+    y2 = [float(v) for v in run(harness, "4 2\n300 2.25\n1200 2.5\n")[:2]]
+    assert y2 == [0.0, 0.0]
