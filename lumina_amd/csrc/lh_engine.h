@@ -153,6 +153,12 @@ class Engine {
   std::vector<Problem> probs_;
   std::vector<SolveDev> sd_;
   std::vector<FarItem> far_free_[2], far_sub_[2];   // [FP64, FP32]
+  // free-space row items with no excluded pair (k_far_row<.., NEAR = false>)
+  std::vector<FarItem> far_clean_[2];
+  DevMem d_far_clean_[2];
+  // bounding boxes of the FP32-rounded positions of every 64-dipole block of
+  // each geometry: lo xyz, hi xyz
+  std::vector<std::vector<double>> box64_;
   std::vector<VecItem> vit_;
   int n_titems_ = 0;
   Tiling tiling_;

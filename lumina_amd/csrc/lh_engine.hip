@@ -22,6 +22,10 @@ void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const
                   const Src64*, double2*, int);
 void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                   const Src32*, double2*, bool, int);
+void launch_far_clean64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                        const Src64*, double2*, int);
+void launch_far_clean32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                        const Src32*, double2*, bool, int);
 void launch_far_sym32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src32*, double2*);
 void launch_far_sym64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
@@ -182,6 +186,21 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   if (!tiling_.hw_sincos && tiling_.block[1] == kSymBlock) tiling_.block[1] = LH_TILE;
   const int S = (int)probs_.size();
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
+
+  box64_.assign(geoms_.size(), {});
+  for (size_t gi = 0; gi < geoms_.size(); ++gi) {
+    const Geometry& g = geoms_[gi];
+    auto& b = box64_[gi];
+    b.assign(6 * (size_t)((g.n + 63) / 64), 0.0);
+    for (int i = 0; i < g.n; ++i) {
+      double* e = &b[6 * (size_t)(i / 64)];
+      const double c[3] = {(double)(float)g.x[i], (double)(float)g.y[i], (double)(float)g.z[i]};
+      for (int a = 0; a < 3; ++a) {
+        if (i % 64 == 0 || c[a] < e[a]) e[a] = c[a];
+        if (i % 64 == 0 || c[a] > e[3 + a]) e[3 + a] = c[a];
+      }
+    }
+  }
 
   std::vector<double> px, py, pz, disp;
   std::vector<int> ecol, etid, gpack, gizj2;
@@ -350,8 +369,32 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   LH_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+// Squared distance between the boxes of the 64-blocks [b0, b1) and [c0, c1).
+double box_gap2(const std::vector<double>& box, int b0, int b1, int c0, int c1) {
+  double lo[2][3], hi[2][3];
+  const int r0[2] = {b0, c0}, r1[2] = {b1, c1};
+  for (int t = 0; t < 2; ++t)
+    for (int a = 0; a < 3; ++a) {
+      lo[t][a] = box[6 * (size_t)r0[t] + a];
+      hi[t][a] = box[6 * (size_t)r0[t] + 3 + a];
+      for (int q = r0[t] + 1; q < r1[t]; ++q) {
+        lo[t][a] = std::min(lo[t][a], box[6 * (size_t)q + a]);
+        hi[t][a] = std::max(hi[t][a], box[6 * (size_t)q + 3 + a]);
+      }
+    }
+  double d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double gap = std::max({0.0, lo[1][a] - hi[0][a], lo[0][a] - hi[1][a]});
+    d2 += gap * gap;
+  }
+  return d2;
+}
+}  // namespace
+
 void Engine::build_far_items(int p) {
   far_free_[p].clear();
+  far_clean_[p].clear();
   far_sub_[p].clear();
   far_ewt_[p].clear();
   far_ewp_[p].clear();
@@ -373,12 +416,26 @@ void Engine::build_far_items(int p) {
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
-        else if (!(sym && free_space)) far_free_[p].push_back(f);
-        else if (ti >= tj) far_free_[p].push_back(f);
+        else if (!(sym && free_space)) {
+          // items whose boxes lie beyond the exclusion radius (with a margin
+          // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
+          // decided per 256 x 256 tile pair, so that every block size sends
+          // the same pairs through the same arithmetic
+          const Geometry& g = geoms_[probs_[s].geom];
+          const double lim = (double)g.rskip2f * (1.0 + 1e-5);
+          const int ta = (f.t0 / LH_TILE) * LH_TILE, tz = std::min(d.n, ta + LH_TILE);
+          const double gap2 = box_gap2(box64_[probs_[s].geom], ta / 64, (tz + 63) / 64,
+                                       f.c0 / 64, (f.c1 + 63) / 64);
+          if (gap2 > lim && gap2 > 0.0) far_clean_[p].push_back(f);
+          else far_free_[p].push_back(f);
+        } else if (ti >= tj) {
+          far_free_[p].push_back(f);
+        }
       }
   }
   drop_graphs();
   upload(d_far_free_[p], far_free_[p], stream_);
+  upload(d_far_clean_[p], far_clean_[p], stream_);
   upload(d_far_sub_[p], far_sub_[p], stream_);
   upload(d_far_ewt_[p], far_ewt_[p], stream_);
   upload(d_far_ewp_[p], far_ewp_[p], stream_);
@@ -571,6 +628,9 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     else
       launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
                    d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+    launch_far_clean32(st, (int)far_clean_[1].size(), d_far_clean_[1].as<FarItem>(), sd, gs,
+                       which, d_src32_.as<Src32>(), d_part_.as<double2>(),
+                       tiling_.hw_sincos != 0, blk);
     launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
                  d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
     launch_ewt_shift32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which,
@@ -586,6 +646,8 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     else
       launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
                    d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    launch_far_clean64(st, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd, gs,
+                       which, d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_ewt_shift64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which,

@@ -226,7 +226,11 @@ struct RowAcc {
 // Far field: every ordered tile pair, row sums; SUB adds the Fresnel image.
 // ---------------------------------------------------------------------------
 
-template <typename R, bool SUB, bool HW>
+// NEAR = false: items whose target and source boxes lie farther apart than
+// the exclusion radius (checked on the host on the FP32 positions), so no
+// pair of the item is excluded and no pair is i == j: the exclusion test
+// drops out and the FP32 |d|^2 takes the FMA form.
+template <typename R, bool SUB, bool HW, bool NEAR = true>
 __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ items,
                                                  const SolveDev* __restrict__ solves,
                                                  const GState* __restrict__ st, int which,
@@ -255,14 +259,21 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
     for (int j = j0; j < j1; ++j) {
       const SRC r = nxt;
       nxt = sp[min(j + 1, it.c1 - 1)];
-      const float d2f = r2f_exact(xif - fx(r), yif - fy(r), zif - fz(r));
       const R dx = xi - r.x, dy = yi - r.y, dz = zi - r.z;
       R Ar, Ai, Br, Bi;
-      if constexpr (std::is_same<R, float>::value) {
+      if constexpr (!NEAR) {
+        if constexpr (std::is_same<R, float>::value)
+          coeffs32<HW>(fma(dz, dz, fma(dy, dy, dx * dx)), false, k, krev, inv_k, c0, Ar, Ai,
+                       Br, Bi);
+        else
+          g_coeffs64(dx * dx + dy * dy + dz * dz, k, inv_k, c0, Ar, Ai, Br, Bi, false);
+      } else if constexpr (std::is_same<R, float>::value) {
         // d2f <= rskip2f also holds for j == i (d2f = 0, rskip2f >= 0); the
         // exclusion test's |d|^2 is the value's |d|^2
+        const float d2f = r2f_exact(xif - fx(r), yif - fy(r), zif - fz(r));
         coeffs32<HW>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
       } else {
+        const float d2f = r2f_exact(xif - fx(r), yif - fy(r), zif - fz(r));
         const bool skip = (d2f <= rskip2f) || (j == i);
         const R r2 = dx * dx + dy * dy + dz * dz;
         g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, skip);
@@ -637,6 +648,26 @@ void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
   else
     hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(block), 0, st, items,
                        solves, gs, which, src, part);
+}
+
+// Free-space items without excluded pairs (k_far_row<R, false, HW, false>).
+void launch_far_clean64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                        const GState* gs, int which, const Src64* src, double2* part, int block) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL((k_far_row<double, false, false, false>), dim3(nitems), dim3(block), 0, st,
+                     items, solves, gs, which, src, part);
+}
+
+void launch_far_clean32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                        const GState* gs, int which, const Src32* src, double2* part,
+                        bool hw_sincos, int block) {
+  if (nitems <= 0) return;
+  if (hw_sincos)
+    hipLaunchKernelGGL((k_far_row<float, false, true, false>), dim3(nitems), dim3(block), 0, st,
+                       items, solves, gs, which, src, part);
+  else
+    hipLaunchKernelGGL((k_far_row<float, false, false, false>), dim3(nitems), dim3(block), 0, st,
+                       items, solves, gs, which, src, part);
 }
 
 void launch_far_sym32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
