@@ -163,16 +163,19 @@ __device__ __forceinline__ void coeffs32(float r2, bool skip, float k, float kre
     sincos_f32(k * r, s, c);
   }
   const float u = inv_r * inv_k;       // 1 / (k r)
-  const float u2 = u * u;
   const float pf = c0 * inv_r;         // k^2 / (4 pi r)
   const float pr = pf * c, pi = pf * s;
-  const float ar = 1.0f - u2, ai = u;
-  const float br = 3.0f * u2 - 1.0f, bi = -3.0f * u;
-  const float ir2 = inv_r * inv_r;
+  const float ar = fmaf(-u, u, 1.0f), ai = u;
+  // b / r^2 with b = (3u^2 - 1) - 3iu, from t2 = 1/r^2 and v3 = 3u/r^2:
+  // (u v3 - t2) - i v3
+  const float t2 = inv_r * inv_r;
+  const float v3 = 3.0f * (t2 * u);
+  const float bi = -v3;
+  const float br = fmaf(u, v3, -t2);
   Ar = pr * ar - pi * ai;
   Ai = pr * ai + pi * ar;
-  Br = (pr * br - pi * bi) * ir2;
-  Bi = (pr * bi + pi * br) * ir2;
+  Br = pr * br - pi * bi;
+  Bi = pr * bi + pi * br;
 }
 
 // out += (A I + B d d^T) v, as fused multiply-add chains into the
@@ -246,7 +249,9 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
   const SRC& me = sp[valid ? i : 0];
   const float xif = fx(me), yif = fy(me), zif = fz(me);
   const R xi = me.x, yi = me.y, zi = me.z;
-  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0, z2 = (R)S.z2;
+  // the prefactor k^2 / (4 pi) is common to every pair: the sums run with
+  // c0 = 1 and the row total is scaled once
+  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = R(1), z2 = (R)S.z2;
   const float krev = (float)(S.k * 0.15915494309189535);
   const float rskip2f = S.rskip2f;
   RowAcc<R> row;
@@ -296,7 +301,8 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
   if (valid) {
     double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = make_double2(row.get(2 * c), row.get(2 * c + 1));
+    for (int c = 0; c < 3; ++c)
+      o[c] = make_double2(row.get(2 * c) * S.c0, row.get(2 * c + 1) * S.c0);
   }
 }
 
