@@ -272,8 +272,14 @@ def run_batch(args):
     achieved = FLOPS_PER_FAR_PAIR * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else 0.0
     peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
     hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
-    kname = (f"k_far_row<float, false, {str(hw).lower()}>" if prec == MIXED
-             else "k_far_row<double, false, false>")
+    # block 512: the one-tensor-per-unordered-pair kernels (with and without
+    # the exclusion test); otherwise the row kernels
+    blk = blocks[1] if prec == MIXED else blocks[0]
+    if blk == 512:
+        kname = "k_far_sym32<true|false>" if prec == MIXED else "k_far_sym<double>"
+    else:
+        kname = (f"k_far_row<float, false, {str(hw).lower()}, true|false>" if prec == MIXED
+                 else "k_far_row<double, false, false, true|false>")
     # HBM bytes per launch of this kernel on this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy)
     traffic = None

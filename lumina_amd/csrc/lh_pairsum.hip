@@ -499,6 +499,189 @@ __global__ __launch_bounds__(256) void k_far_sym(const FarItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
+// FP32 far field, one tensor per unordered pair, sources staged in LDS.
+//
+// Same items and pairing as k_far_sym (T >= S, wave w holds targets
+// T*256 + 64 w + lane), but the four 64-source sub-blocks of S are taken one
+// at a time by the whole block: sub-block b is staged in LDS twice over
+// (entry e holds source e mod 64), so at step s lane l reads source
+// (l + s) mod 64 at entry l + s with no wrap-around. Only the transposed
+// accumulators move between lanes: each step's transposed contribution is
+// added to the accumulator of lane l + 1 (v_add_f32 with DPP wave_rol:1), so
+// after step s lane l holds the sum of source (l + s) mod 64. The next step's
+// source is read from LDS while the current one is used, and a scheduling
+// barrier keeps the steps apart, which holds the register count to what one
+// step in flight needs.
+// Per unordered pair: 27 VALU for the tensor coefficients and 2 x 28 for the
+// two products (31 with the exclusion test), against 2 x 55 for the two
+// ordered pairs in the row kernel. The prefactor k^2 / 4 pi is applied once
+// per output. After each sub-block the four waves' transposed sums meet in
+// LDS and wave b adds them in wave order, in FP64: results are deterministic.
+// NEAR = false: tile pairs whose boxes lie beyond the exclusion radius (no
+// excluded pair, no coincident pair, never the diagonal), |d|^2 by FMA.
+// ---------------------------------------------------------------------------
+
+// a + (the value of lane l + 1; lane 63 takes lane 0's): DPP wave_rol:1 on the
+// first operand, written so that the move folds into v_add_f32_dpp (old value
+// 0, the identity of the add, all rows and banks enabled)
+__device__ __forceinline__ float rol1_add(float a, float t) {
+  const float r = __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, a), 0x134, 0xF, 0xF, false));
+  return r + t;
+}
+
+template <bool NEAR>
+__global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ items,
+                                                   const SolveDev* __restrict__ solves,
+                                                   const GState* __restrict__ st, int which,
+                                                   const Src32* __restrict__ src,
+                                                   double2* __restrict__ part) {
+  __shared__ float4 s_a[128];        // x, y, z, v0 of source (e mod 64) of the sub-block
+  __shared__ float4 s_b[128];        // v1 .. v4
+  __shared__ float s_c[128];         // v5
+  __shared__ float s_t[4][64][6];    // [wave][source lane][component]: transposed sums
+  const FarItem it = items[blockIdx.x];
+  if (!takes_part(st, it.s, which)) return;
+  const SolveDev S = solves[it.s];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int i = it.t0 + 64 * w + lane;
+  const bool tvalid = i < S.n;
+  // a wave with no valid target skips the pair work; its missing targets
+  // carry x = 0 and the position of the wave's first target
+  const bool live = it.t0 + 64 * w < S.n;
+  const Src32* __restrict__ sp = src + S.src_off;
+  const Src32& me = sp[tvalid ? i : (live ? it.t0 + 64 * w : it.t0)];
+  const float xi = me.x, yi = me.y, zi = me.z;
+  float vi[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0f;
+  const float k = (float)S.k, inv_k = (float)S.inv_k;
+  const float krev = (float)(S.k * 0.15915494309189535);
+  const float rskip2f = S.rskip2f;
+  const bool diag = NEAR && it.t0 == it.c0;
+  const double c0 = S.c0;
+  const long long n3 = 3LL * S.n;
+  float hi[6], lo[6], run[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = 0.0f;
+  double tr[6] = {0, 0, 0, 0, 0, 0};   // diagonal item: transposed sum of target i
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      two_sum(hi[q], lo[q], run[q]);
+      run[q] = 0.0f;
+    }
+  };
+
+  for (int b = 0; b < 4; ++b) {
+    const int jb = it.c0 + 64 * b;
+    if (jb >= S.n) break;   // uniform over the block
+    if (tid < 128) {
+      // missing sources: x = 0 at the position of the sub-block's first source
+      const int j = jb + (tid & 63);
+      const bool sv = j < S.n;
+      const Src32& r = sp[sv ? j : jb];
+      s_a[tid] = make_float4(r.x, r.y, r.z, sv ? r.v[0] : 0.0f);
+      s_b[tid] = sv ? make_float4(r.v[1], r.v[2], r.v[3], r.v[4]) : make_float4(0, 0, 0, 0);
+      s_c[tid] = sv ? r.v[5] : 0.0f;
+    }
+    __syncthreads();
+    float ta[6] = {0, 0, 0, 0, 0, 0};
+    int home = 0;   // lane l ends holding the transposed sum of source (l + home) mod 64
+
+    // steps s0 .. s1 - 1, both directions; the source of step s + 1 is read
+    // while step s computes
+    auto steps = [&](int s0, int s1) {
+      float4 ca = s_a[lane + s0], cb = s_b[lane + s0];
+      float cc = s_c[lane + s0];
+#pragma unroll 4
+      for (int s = s0; s < s1; ++s) {
+        // entry lane + s + 1 <= 127
+        const float4 na = s_a[lane + s + 1], nb = s_b[lane + s + 1];
+        const float nc = s_c[lane + s + 1];
+        const float vs[6] = {ca.w, cb.x, cb.y, cb.z, cb.w, cc};
+        const float dx = xi - ca.x, dy = yi - ca.y, dz = zi - ca.z;
+        float Ar, Ai, Br, Bi;
+        if constexpr (NEAR) {
+          const float d2f = r2f_exact(dx, dy, dz);
+          coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
+        } else {
+          coeffs32<true>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), false, k, krev, inv_k, 1.0f, Ar,
+                         Ai, Br, Bi);
+        }
+        apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+        float t6[6] = {0, 0, 0, 0, 0, 0};
+        apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ta[q] = rol1_add(ta[q], t6[q]);
+        ca = na;
+        cb = nb;
+        cc = nc;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+
+    if (live && !(diag && b < w)) {
+      if (diag && b == w) {
+        // steps 1..31 both ways; step 32 pairs (l, l + 32) twice over, so
+        // each lane adds its own direction only
+        steps(1, 32);
+        {
+          const int e = lane + 32;
+          const float4 A4 = s_a[e], B4 = s_b[e];
+          const float vs[6] = {A4.w, B4.x, B4.y, B4.z, B4.w, s_c[e]};
+          const float dx = xi - A4.x, dy = yi - A4.y, dz = zi - A4.z;
+          const float d2f = r2f_exact(dx, dy, dz);
+          float Ar, Ai, Br, Bi;
+          coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
+          apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+        }
+        fold();
+        home = 31;
+      } else {
+        steps(0, 32);
+        fold();
+        steps(32, 64);
+        fold();
+        home = 63;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_t[w][(lane + home) & 63][q] = ta[q];
+    __syncthreads();
+    // the staged sources are no longer read: the next sub-block may be staged
+    // while wave b sums this one's transposed sums (read before the next
+    // barrier, written after it)
+    if (w == b) {
+      double t[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        t[q] = ((double)s_t[0][lane][q] + (double)s_t[1][lane][q]) +
+               ((double)s_t[2][lane][q] + (double)s_t[3][lane][q]);
+      if (diag) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) tr[q] = t[q];
+      } else if (jb + lane < S.n) {
+        // transposed sums of source jb + lane go to the target tile's slice
+        double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = make_double2(t[2 * c] * c0, t[2 * c + 1] * c0);
+      }
+    }
+  }
+  if (tvalid) {
+    // forward sums (plus, on the diagonal, the transposed sums of the same
+    // dipole) to the source tile's slice
+    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      o[c] = make_double2((((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
+                          (((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]) * c0);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reduction: y_i = alpha_i^-1 x_i - sum_tiles part - sum_near T_ij x_j.
 // mode 0 writes y to slot `out_slot`; mode 1 writes r = rhs - y to the
 // residual slot and a per-item partial of |r|^2 to norm_part.
@@ -676,11 +859,17 @@ void launch_far_clean32(hipStream_t st, int nitems, const FarItem* items, const 
                        items, solves, gs, which, src, part);
 }
 
-void launch_far_sym32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
-                      const GState* gs, int which, const Src32* src, double2* part) {
+// near = false: items without excluded pairs (k_far_sym32<false>).
+void launch_far_sym32(hipStream_t st, bool near, int nitems, const FarItem* items,
+                      const SolveDev* solves, const GState* gs, int which, const Src32* src,
+                      double2* part) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL((k_far_sym<float>), dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
-                     src, part);
+  if (near)
+    hipLaunchKernelGGL((k_far_sym32<true>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
+                       which, src, part);
+  else
+    hipLaunchKernelGGL((k_far_sym32<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
+                       which, src, part);
 }
 
 void launch_far_sym64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,

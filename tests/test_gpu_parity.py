@@ -288,7 +288,9 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             for q in range(len(it)):
                 if it0[q] < 200:
                     assert abs(it[q] - it0[q]) <= 2, (name, q)
-                    assert np.all(np.abs(cs[q] - cs0[q]) <= 1e-9 * np.abs(cs0[q])), (name, q)
+                    # C_sca = C_ext - C_abs carries their absolute error: the
+                    # bar scales with C_ext for all three
+                    assert np.all(np.abs(cs[q] - cs0[q]) <= 1e-9 * np.abs(cs0[q][0])), (name, q)
 
 
 @pytest.mark.parametrize("n", [1, 2, 65, 257, 515, 700])
@@ -339,6 +341,31 @@ def test_matvec_fp32_symmetric_kernel(backend, n, monkeypatch):
     e = rel(s.matvec(x, MIXED), yo)
     print(f"sym fp32 matvec n={len(pos)} rel err {e:.2e}")
     assert e < 2e-6
+
+
+@pytest.mark.parametrize("nx", [40, 97])
+def test_matvec_fp32_symmetric_kernel_far_tiles(backend, nx, monkeypatch):
+    """A rod of nx x 4 x 4 dipoles ordered along its axis: 256-dipole tiles
+    two or more apart lie beyond the exclusion radius, so the FP32
+    one-tensor-per-pair kernel runs in both forms (with and without the
+    exclusion test), on a ragged last tile for nx = 97. Against the oracle
+    and against the row kernel."""
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(4), np.arange(4), indexing="ij"), -1)
+    pos = 1.5 * g.reshape(-1, 3).astype(float)
+    alpha = I.isotropic_block(40.0 + 10.0j)
+    k = 2 * np.pi / 480.0
+    x = rand_x(len(pos), 12)
+    yo = O.Problem(pos, alpha, k, True, 1.5).matvec(x)
+    out = {}
+    for blk in ("512", "256"):
+        monkeypatch.setenv("LUMINA_FAR_BLOCK", blk)
+        s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.5)
+        out[blk] = s.matvec(x, MIXED)
+        s.close()
+    e, e_row = rel(out["512"], yo), rel(out["512"], out["256"])
+    print(f"sym fp32 rod n={len(pos)} rel err {e:.2e} vs row kernel {e_row:.2e}")
+    assert e < 2e-6
+    assert e_row < 2e-6
 
 
 def test_stored_matrix_session(backend):
