@@ -195,6 +195,23 @@ __device__ __forceinline__ void apply(R dx, R dy, R dz, R Ar, R Ai, R Br, R Bi, 
   out[5] = fma(dz, wi, fma(Ai, v[4], fma(Ar, v[5], out[5])));
 }
 
+// out = (A I + B d d^T) v, starting each component with a multiply (a fresh
+// sum, not an accumulation onto zero).
+template <typename R>
+__device__ __forceinline__ void apply_set(R dx, R dy, R dz, R Ar, R Ai, R Br, R Bi, const R* v,
+                                          R out[6]) {
+  const R dr = fma(dz, v[4], fma(dy, v[2], dx * v[0]));
+  const R di = fma(dz, v[5], fma(dy, v[3], dx * v[1]));
+  const R wr = fma(-Bi, di, Br * dr);
+  const R wi = fma(Bi, dr, Br * di);
+  out[0] = fma(dx, wr, fma(-Ai, v[1], Ar * v[0]));
+  out[1] = fma(dx, wi, fma(Ai, v[0], Ar * v[1]));
+  out[2] = fma(dy, wr, fma(-Ai, v[3], Ar * v[2]));
+  out[3] = fma(dy, wi, fma(Ai, v[2], Ar * v[3]));
+  out[4] = fma(dz, wr, fma(-Ai, v[5], Ar * v[4]));
+  out[5] = fma(dz, wi, fma(Ai, v[4], Ar * v[5]));
+}
+
 // Error-free TwoSum of hi + v into (hi, lo).
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float v) {
   const float s = hi + v;
@@ -611,8 +628,8 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
                          Ai, Br, Bi);
         }
         apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
-        float t6[6] = {0, 0, 0, 0, 0, 0};
-        apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
+        float t6[6];
+        apply_set<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
 #pragma unroll
         for (int q = 0; q < 6; ++q) ta[q] = rol1_add(ta[q], t6[q]);
         ca = na;
