@@ -324,7 +324,8 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
-// Far field, one tensor per unordered pair (free-space solves), FP32 or FP64.
+// Far field, one tensor per unordered pair (free-space solves), FP64 (the
+// FP32 form, k_far_sym32, stages its sources in LDS instead).
 //
 // G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
 // evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
@@ -344,11 +345,8 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 // radius and skipped).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ float wave_ror1(float v) {
-  // DPP wave_ror:1 (0x13C): lane l takes the value of lane l - 1 (lane 0 from 63)
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
-}
+// DPP wave_ror:1 (0x13C) on both halves: lane l takes the value of lane l - 1
+// (lane 0 from 63)
 __device__ __forceinline__ double wave_ror1(double v) {
   const long long b = __builtin_bit_cast(long long, v);
   const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
@@ -356,15 +354,9 @@ __device__ __forceinline__ double wave_ror1(double v) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// One pair (target lane, source rotated into the lane): the FP32 form uses
-// the FP32 exclusion distance as |d|^2, the FP64 form evaluates |d|^2 in FP64
-// and takes the exclusion decision on the FP32-rounded positions.
+// One pair (target lane, source rotated into the lane): |d|^2 in FP64, the
+// exclusion decision on the FP32-rounded positions.
 template <typename R> struct SymPos;
-template <> struct SymPos<float> {
-  float x, y, z;
-  __device__ void set(const Src32& r) { x = r.x; y = r.y; z = r.z; }
-  __device__ void rotate() { x = wave_ror1(x); y = wave_ror1(y); z = wave_ror1(z); }
-};
 template <> struct SymPos<double> {
   double x, y, z;
   __device__ void set(const Src64& r) { x = r.x; y = r.y; z = r.z; }
@@ -378,15 +370,10 @@ __device__ __forceinline__ void sym_coeffs(const SymPos<R>& t, const SymPos<R>& 
   dx = t.x - sp.x;
   dy = t.y - sp.y;
   dz = t.z - sp.z;
-  if constexpr (std::is_same<R, float>::value) {
-    const float d2f = r2f_exact(dx, dy, dz);
-    coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, c0, Ar, Ai, Br, Bi);
-  } else {
-    const float d2f = r2f_exact((float)t.x - (float)sp.x, (float)t.y - (float)sp.y,
-                                (float)t.z - (float)sp.z);
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, d2f <= rskip2f);
-  }
+  const float d2f = r2f_exact((float)t.x - (float)sp.x, (float)t.y - (float)sp.y,
+                              (float)t.z - (float)sp.z);
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, d2f <= rskip2f);
 }
 
 template <typename R>
