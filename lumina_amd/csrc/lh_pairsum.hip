@@ -8,11 +8,14 @@
 //  * far field: every pair outside the exclusion radius (and j != i)
 //    evaluates the point Green's tensor (greens.rs:34-71) on the fly in the
 //    y += A x + B d (d . x) form -- no tensor is materialised. One target
-//    dipole per lane; source records stream through the scalar cache (every
-//    lane of a wave reads the same record), so a source costs no LDS traffic.
-//    Work items are tiles of (64, 128 or 256 targets) x (256 sources); each
-//    writes the row sums of its targets to the partial slice of its source
-//    tile (no atomics, reproducible results). With a substrate (k_far_row<R, true>,
+//    dipole per lane. In the row form (k_far_row) source records stream
+//    through the scalar cache (every lane of a wave reads the same record);
+//    work items are tiles of (64, 128 or 256 targets) x (256 sources), each
+//    writing the row sums of its targets to the partial slice of its source
+//    tile (no atomics, reproducible results). The free-space forms with one
+//    tensor per unordered pair (k_far_sym32 in FP32, sources staged in LDS;
+//    k_far_sym in FP64) also write the column sums of their sources to the
+//    slice of their target tile. With a substrate (k_far_row<R, true>,
 //    assembly.rs:272-295) the image tensor at the mirrored source is applied
 //    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
 //    |d|^2 < 1e-20.
