@@ -282,6 +282,12 @@ int lh_backend_device_info(const lh_backend* b, lh_device_info* out) {
     std::memset(out, 0, sizeof(*out));
     std::strncpy(out->name, b->prop.name, sizeof(out->name) - 1);
     std::strncpy(out->arch, b->prop.gcnArchName, sizeof(out->arch) - 1);
+    if (out->name[0] == '\0') {
+      // some drivers report no marketing name: name the device by its ISA
+      const std::string isa(b->prop.gcnArchName);
+      const std::string nm = "AMD GPU " + isa.substr(0, isa.find(':'));
+      std::strncpy(out->name, nm.c_str(), sizeof(out->name) - 1);
+    }
     out->backend_type = 1;
     out->memory_bytes = (uint64_t)b->prop.totalGlobalMem;
     out->compute_units = (uint32_t)b->prop.multiProcessorCount;

@@ -37,6 +37,8 @@ def test_device_info(backend):
     print(info)
     assert info["backend_type"] == "Gpu"
     assert "gfx950" in info["arch"]
+    assert info["name"]
+    assert info["compute_units"] == 256
 
 
 @pytest.mark.parametrize("use_fcd", [True, False])
