@@ -28,7 +28,7 @@ void launch_far_clean32(hipStream_t, int, const FarItem*, const SolveDev*, const
                         const Src32*, double2*, bool, int);
 void launch_far_sym32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src32*, double2*);
-void launch_far_sym64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+void launch_far_sym64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
@@ -416,12 +416,12 @@ void Engine::build_far_items(int p) {
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
-        else if (!(sym && free_space) || (ti >= tj && p == 1)) {
+        else if (!(sym && free_space) || ti >= tj) {
           // items whose boxes lie beyond the exclusion radius (with a margin
           // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
           // decided per 256 x 256 tile pair, so that every block size sends
-          // the same pairs through the same arithmetic (the FP32
-          // one-tensor-per-pair kernel splits its items T >= S the same way)
+          // the same pairs through the same arithmetic (the
+          // one-tensor-per-pair kernels split their items T >= S the same way)
           const Geometry& g = geoms_[probs_[s].geom];
           const double lim = (double)g.rskip2f * (1.0 + 1e-5);
           const int ta = (f.t0 / LH_TILE) * LH_TILE, tz = std::min(d.n, ta + LH_TILE);
@@ -644,14 +644,17 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    if (sym)
-      launch_far_sym64(st, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
+    if (sym) {
+      launch_far_sym64(st, true, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
                        d_src64_.as<Src64>(), d_part_.as<double2>());
-    else
+      launch_far_sym64(st, false, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd,
+                       gs, which, d_src64_.as<Src64>(), d_part_.as<double2>());
+    } else {
       launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
                    d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
-    launch_far_clean64(st, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd, gs,
-                       which, d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+      launch_far_clean64(st, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd, gs,
+                         which, d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+    }
     launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
                  d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
     launch_ewt_shift64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which,

@@ -14,7 +14,7 @@
 //    writing the row sums of its targets to the partial slice of its source
 //    tile (no atomics, reproducible results). The free-space forms with one
 //    tensor per unordered pair (k_far_sym32 in FP32, sources staged in LDS;
-//    k_far_sym in FP64) also write the column sums of their sources to the
+//    k_far_sym64 in FP64) also write the column sums of their sources to the
 //    slice of their target tile. With a substrate (k_far_row<R, true>,
 //    assembly.rs:272-295) the image tensor at the mirrored source is applied
 //    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
@@ -327,189 +327,19 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 }
 
 // ---------------------------------------------------------------------------
-// Far field, one tensor per unordered pair (free-space solves), FP64 (the
-// FP32 form, k_far_sym32, stages its sources in LDS instead).
+// FP32 far field, one tensor per unordered pair, sources staged in LDS.
 //
 // G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
 // evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
 // item is a (target tile T, source tile S) pair with T >= S, 256 threads:
-// wave w holds targets T*256 + 64 w + lane and walks the four 64-source
-// sub-blocks of S. Each lane starts with its own source and the sources, with
-// their transposed accumulators, rotate one lane per step around the wave
-// (DPP wave_ror:1), so after 64 steps every target has met every source of the
-// sub-block and every accumulator is back in its lane. On the diagonal tile
-// (T == S) wave w takes sub-blocks b >= w only, and its own sub-block b == w
-// for steps 1..32 (step 32 one way only), so each unordered pair is counted
-// once. Transposed sums are combined over the four waves in LDS in a fixed
-// order: results are deterministic. Forward sums of T go to partial slice S,
-// transposed sums of S to slice T (on the diagonal both go to slice T).
-// Missing sources of a partial tile carry x = 0 and a copy of a real position:
-// their contribution is exactly 0 (a coincident pair is inside the exclusion
-// radius and skipped).
-// ---------------------------------------------------------------------------
-
-// DPP wave_ror:1 (0x13C) on both halves: lane l takes the value of lane l - 1
-// (lane 0 from 63)
-__device__ __forceinline__ double wave_ror1(double v) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x13C, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// One pair (target lane, source rotated into the lane): |d|^2 in FP64, the
-// exclusion decision on the FP32-rounded positions.
-template <typename R> struct SymPos;
-template <> struct SymPos<double> {
-  double x, y, z;
-  __device__ void set(const Src64& r) { x = r.x; y = r.y; z = r.z; }
-  __device__ void rotate() { x = wave_ror1(x); y = wave_ror1(y); z = wave_ror1(z); }
-};
-
-template <typename R>
-__device__ __forceinline__ void sym_coeffs(const SymPos<R>& t, const SymPos<R>& sp, float rskip2f,
-                                           R k, float krev, R inv_k, R c0, R& dx, R& dy, R& dz,
-                                           R& Ar, R& Ai, R& Br, R& Bi) {
-  dx = t.x - sp.x;
-  dy = t.y - sp.y;
-  dz = t.z - sp.z;
-  const float d2f = r2f_exact((float)t.x - (float)sp.x, (float)t.y - (float)sp.y,
-                              (float)t.z - (float)sp.z);
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, d2f <= rskip2f);
-}
-
-template <typename R>
-__global__ __launch_bounds__(256) void k_far_sym(const FarItem* __restrict__ items,
-                                                 const SolveDev* __restrict__ solves,
-                                                 const GState* __restrict__ st, int which,
-                                                 const typename SrcOf<R>::type* __restrict__ src,
-                                                 double2* __restrict__ part) {
-  typedef typename SrcOf<R>::type SRC;
-  __shared__ R lds_t[4][4][64][6];   // [wave][sub-block][source lane][component]
-  const FarItem it = items[blockIdx.x];
-  if (!takes_part(st, it.s, which)) return;
-  const SolveDev S = solves[it.s];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = it.t0 + 64 * w + lane;
-  const bool tvalid = i < S.n;
-  const SRC* __restrict__ sp = src + S.src_off;
-  const SRC me = sp[tvalid ? i : 0];
-  SymPos<R> tp;
-  tp.set(me);
-  R vi[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : R(0);
-  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0;
-  const float krev = (float)(S.k * 0.15915494309189535);
-  const float rskip2f = S.rskip2f;
-  const bool diag = it.t0 == it.c0;
-  R hi[6], lo[6], run[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = R(0);
-
-  auto fold = [&]() {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      if constexpr (std::is_same<R, float>::value) two_sum(hi[q], lo[q], run[q]);
-      else hi[q] += run[q];
-      run[q] = R(0);
-    }
-  };
-
-  for (int b = 0; b < 4; ++b) {
-    const int jb = it.c0 + 64 * b;
-    R ta[6] = {0, 0, 0, 0, 0, 0};
-    if (jb >= S.n || (diag && b < w)) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) lds_t[w][b][lane][q] = R(0);
-      continue;
-    }
-    const int j = jb + lane;
-    const bool sv = j < S.n;
-    const SRC rs = sp[sv ? j : jb];
-    SymPos<R> spos;
-    spos.set(rs);
-    R vs[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) vs[q] = sv ? rs.v[q] : R(0);
-
-    // one unordered pair per lane: forward into run, transposed into ta
-    auto step = [&](bool both) {
-      R dx, dy, dz, Ar, Ai, Br, Bi;
-      sym_coeffs<R>(tp, spos, rskip2f, k, krev, inv_k, c0, dx, dy, dz, Ar, Ai, Br, Bi);
-      apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
-      if (both) apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
-    };
-    auto rotate = [&]() {
-      spos.rotate();
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        vs[q] = wave_ror1(vs[q]);
-        ta[q] = wave_ror1(ta[q]);
-      }
-    };
-    int home;   // rotation of the accumulators at the end
-    if (diag && b == w) {
-      for (int kk = 1; kk < 32; ++kk) {
-        rotate();
-        step(true);
-      }
-      rotate();
-      step(false);   // pairs (l, l - 32): each lane adds its own direction
-      fold();
-      home = 32;
-    } else {
-      for (int kk = 0; kk < 64; ++kk) {
-        step(true);
-        rotate();
-        if (kk == 31) fold();
-      }
-      fold();
-      home = 0;
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) lds_t[w][b][(lane - home) & 63][q] = ta[q];
-  }
-  __syncthreads();
-  // thread (w, lane): forward sum of target i; transposed sum of source
-  // c0 + 64 w + lane over the four waves
-  double tr[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q)
-    tr[q] = ((double)lds_t[0][w][lane][q] + (double)lds_t[1][w][lane][q]) +
-            ((double)lds_t[2][w][lane][q] + (double)lds_t[3][w][lane][q]);
-  const long long n3 = 3LL * S.n;
-  if (diag) {
-    if (tvalid) {
-      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        o[c] = make_double2(((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c],
-                            ((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]);
-    }
-  } else {
-    if (tvalid) {
-      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        o[c] = make_double2((double)hi[2 * c] + (double)lo[2 * c],
-                            (double)hi[2 * c + 1] + (double)lo[2 * c + 1]);
-    }
-    const int jj = it.c0 + 64 * w + lane;
-    if (jj < S.n) {
-      double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * jj;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o[c] = make_double2(tr[2 * c], tr[2 * c + 1]);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 far field, one tensor per unordered pair, sources staged in LDS.
-//
-// Same items and pairing as k_far_sym (T >= S, wave w holds targets
-// T*256 + 64 w + lane), but the four 64-source sub-blocks of S are taken one
+// wave w holds targets T*256 + 64 w + lane. On the diagonal tile (T == S)
+// wave w takes sub-blocks b >= w only, and its own sub-block for steps 1..32
+// (step 32 one way only), so each unordered pair is counted once. Forward
+// sums of T go to partial slice S, transposed sums of S to slice T (on the
+// diagonal both go to slice T). Missing sources of a partial tile carry
+// x = 0 and a copy of a real position: their contribution is exactly 0 (a
+// coincident pair is inside the exclusion radius and skipped).
+// The four 64-source sub-blocks of S are taken one
 // at a time by the whole block: sub-block b is staged in LDS twice over
 // (entry e holds source e mod 64), so at step s lane l reads source
 // (l + s) mod 64 at entry l + s with no wrap-around. Only the transposed
@@ -685,6 +515,167 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
     for (int c = 0; c < 3; ++c)
       o[c] = make_double2((((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
                           (((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]) * c0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 far field, one tensor per unordered pair, sources staged in LDS.
+//
+// The scheme of k_far_sym32 in FP64: items T >= S, wave w holds targets
+// T*256 + 64 w + lane, each 64-source sub-block of S staged in LDS twice over
+// and read one step ahead at entry lane + s. DPP has no 64-bit add, so the
+// transposed accumulators are moved first (two 32-bit wave_rol:1 moves per
+// double, 12 per step) and the step's contribution is then accumulated onto
+// them; after step s lane l holds the sum of source (l + s) mod 64. The
+// exclusion test uses the FP32-rounded positions (the same r2f_exact test as
+// every other kernel and the near list); NEAR = false for tile pairs whose
+// boxes lie beyond the exclusion radius. Sums run in FP64; the prefactor
+// k^2 / 4 pi is applied once per output. After each sub-block the waves'
+// transposed sums meet in LDS and the owning wave adds them in wave order.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_rol1(double v) {
+  // DPP wave_rol:1 (0x134) on both halves: lane l takes lane l + 1's value
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <bool NEAR>
+__global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ items,
+                                                   const SolveDev* __restrict__ solves,
+                                                   const GState* __restrict__ st, int which,
+                                                   const Src64* __restrict__ src,
+                                                   double2* __restrict__ part) {
+  // one stride for the five arrays: a step's reads share an address register
+  __shared__ double2 s_a[128];       // x, y of source (e mod 64) of the sub-block
+  __shared__ double2 s_b[128];       // z, v0
+  __shared__ double2 s_c[128];       // v1, v2
+  __shared__ double2 s_d[128];       // v3, v4
+  __shared__ double2 s_e[128];       // v5, unused
+  __shared__ double s_t[4][64][6];   // [wave][source lane][component]: transposed sums
+  const FarItem it = items[blockIdx.x];
+  if (!takes_part(st, it.s, which)) return;
+  const SolveDev S = solves[it.s];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int i = it.t0 + 64 * w + lane;
+  const bool tvalid = i < S.n;
+  // a wave with no valid target skips the pair work; its missing targets
+  // carry x = 0 and the position of the wave's first target
+  const bool live = it.t0 + 64 * w < S.n;
+  const Src64* __restrict__ sp = src + S.src_off;
+  const Src64& me = sp[tvalid ? i : (live ? it.t0 + 64 * w : it.t0)];
+  const double xi = me.x, yi = me.y, zi = me.z;
+  const float xif = me.xf, yif = me.yf, zif = me.zf;
+  double vi[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0;
+  const double k = S.k, inv_k = S.inv_k;
+  const float rskip2f = S.rskip2f;
+  const bool diag = NEAR && it.t0 == it.c0;
+  const double c0 = S.c0;
+  const long long n3 = 3LL * S.n;
+  double run[6] = {0, 0, 0, 0, 0, 0};
+  double tr[6] = {0, 0, 0, 0, 0, 0};   // diagonal item: transposed sum of target i
+
+  for (int b = 0; b < 4; ++b) {
+    const int jb = it.c0 + 64 * b;
+    if (jb >= S.n) break;   // uniform over the block
+    if (tid < 128) {
+      // missing sources: x = 0 at the position of the sub-block's first source
+      const int j = jb + (tid & 63);
+      const bool sv = j < S.n;
+      const Src64& r = sp[sv ? j : jb];
+      s_a[tid] = make_double2(r.x, r.y);
+      s_b[tid] = make_double2(r.z, sv ? r.v[0] : 0.0);
+      s_c[tid] = sv ? make_double2(r.v[1], r.v[2]) : make_double2(0.0, 0.0);
+      s_d[tid] = sv ? make_double2(r.v[3], r.v[4]) : make_double2(0.0, 0.0);
+      s_e[tid] = make_double2(sv ? r.v[5] : 0.0, 0.0);
+    }
+    __syncthreads();
+    double ta[6] = {0, 0, 0, 0, 0, 0};
+    int home = 0;   // lane l ends holding the transposed sum of source (l + home) mod 64
+
+    // one unordered pair: target of this lane, the source held in a..e;
+    // both = false adds the forward direction only
+    auto pair = [&](double2 a, double2 bb, double2 c, double2 d, double2 e, bool both) {
+      const double vs[6] = {bb.y, c.x, c.y, d.x, d.y, e.x};
+      const double dx = xi - a.x, dy = yi - a.y, dz = zi - bb.x;
+      double Ar, Ai, Br, Bi;
+      if constexpr (NEAR) {
+        const float d2f = r2f_exact(xif - (float)a.x, yif - (float)a.y, zif - (float)bb.x);
+        g_coeffs64(dx * dx + dy * dy + dz * dz, k, inv_k, 1.0, Ar, Ai, Br, Bi, d2f <= rskip2f);
+      } else {
+        g_coeffs64(dx * dx + dy * dy + dz * dz, k, inv_k, 1.0, Ar, Ai, Br, Bi, false);
+      }
+      apply<double>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+      if (both) apply<double>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
+    };
+    // steps s0 .. s1 - 1, both directions; the source of step s + 1 is read
+    // while step s computes
+    auto steps = [&](int s0, int s1) {
+      double2 ca = s_a[lane + s0], cb = s_b[lane + s0], cc = s_c[lane + s0],
+              cd = s_d[lane + s0], ce = s_e[lane + s0];
+#pragma unroll 2
+      for (int s = s0; s < s1; ++s) {
+        // entry lane + s + 1 <= 127
+        const int e1 = lane + s + 1;
+        const double2 na = s_a[e1], nb = s_b[e1], nc = s_c[e1], nd = s_d[e1], ne = s_e[e1];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ta[q] = wave_rol1(ta[q]);
+        pair(ca, cb, cc, cd, ce, true);
+        ca = na;
+        cb = nb;
+        cc = nc;
+        cd = nd;
+        ce = ne;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+
+    if (live && !(diag && b < w)) {
+      if (diag && b == w) {
+        // steps 1..31 both ways; step 32 pairs (l, l + 32) twice over, so
+        // each lane adds its own direction only
+        steps(1, 32);
+        const int e = lane + 32;
+        pair(s_a[e], s_b[e], s_c[e], s_d[e], s_e[e], false);
+        home = 31;
+      } else {
+        steps(0, 64);
+        home = 63;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_t[w][(lane + home) & 63][q] = ta[q];
+    __syncthreads();
+    // the staged sources are no longer read: the next sub-block may be staged
+    // while wave b sums this one's transposed sums (read before the next
+    // barrier, written after it)
+    if (w == b) {
+      double t[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        t[q] = (s_t[0][lane][q] + s_t[1][lane][q]) + (s_t[2][lane][q] + s_t[3][lane][q]);
+      if (diag) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) tr[q] = t[q];
+      } else if (jb + lane < S.n) {
+        // transposed sums of source jb + lane go to the target tile's slice
+        double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = make_double2(t[2 * c] * c0, t[2 * c + 1] * c0);
+      }
+    }
+  }
+  if (tvalid) {
+    // forward sums (plus, on the diagonal, the transposed sums of the same
+    // dipole) to the source tile's slice
+    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      o[c] = make_double2((run[2 * c] + tr[2 * c]) * c0, (run[2 * c + 1] + tr[2 * c + 1]) * c0);
   }
 }
 
@@ -879,11 +870,17 @@ void launch_far_sym32(hipStream_t st, bool near, int nitems, const FarItem* item
                        which, src, part);
 }
 
-void launch_far_sym64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
-                      const GState* gs, int which, const Src64* src, double2* part) {
+// near = false: items without excluded pairs (k_far_sym64<false>).
+void launch_far_sym64(hipStream_t st, bool near, int nitems, const FarItem* items,
+                      const SolveDev* solves, const GState* gs, int which, const Src64* src,
+                      double2* part) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL((k_far_sym<double>), dim3(nitems), dim3(256), 0, st, items, solves, gs, which,
-                     src, part);
+  if (near)
+    hipLaunchKernelGGL((k_far_sym64<true>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
+                       which, src, part);
+  else
+    hipLaunchKernelGGL((k_far_sym64<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
+                       which, src, part);
 }
 
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
