@@ -272,11 +272,14 @@ def run_batch(args):
     achieved = FLOPS_PER_FAR_PAIR * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else 0.0
     peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
     hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
-    # block 512: the one-tensor-per-unordered-pair kernels (with and without
-    # the exclusion test); otherwise the row kernels
+    # block 512: the one-tensor-per-unordered-pair kernels with LDS-staged
+    # sources (with and without the exclusion test); 1024: the FP64 one whose
+    # sources rotate around the wave; otherwise the row kernels
     blk = blocks[1] if prec == MIXED else blocks[0]
     if blk == 512:
-        kname = "k_far_sym32<true|false>" if prec == MIXED else "k_far_sym<double>"
+        kname = "k_far_sym32<true|false>" if prec == MIXED else "k_far_sym64<true|false>"
+    elif blk == 1024:
+        kname = "k_far_sym<double>"
     else:
         kname = (f"k_far_row<float, false, {str(hw).lower()}, true|false>" if prec == MIXED
                  else "k_far_row<double, false, false, true|false>")

@@ -197,6 +197,10 @@ lh::Tiling tiling_from_env() {
       t.block[0] = 256;
       t.block[1] = lh::kSymBlock;
     }
+    if (b == lh::kSymRotBlock) {
+      t.block[0] = lh::kSymRotBlock;
+      t.block[1] = 256;
+    }
   }
   return t;
 }

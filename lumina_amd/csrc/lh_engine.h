@@ -33,14 +33,17 @@ struct DevMem {
 };
 
 // Far-field launch options. block[] is the autotuner's search space: 64, 128
-// or 256 targets per work item of the row kernels, or kSymBlock for the
-// kernels that evaluate one tensor per unordered pair.
+// or 256 targets per work item of the row kernels, kSymBlock for the kernels
+// that evaluate one tensor per unordered pair with LDS-staged sources, or
+// (FP64 only) kSymRotBlock for the unordered-pair kernel whose sources rotate
+// around the wave.
 constexpr int kSymBlock = 512;
+constexpr int kSymRotBlock = 1024;
 struct Tiling {
   int hw_sincos = 1;          // FP32 far field: hardware v_sin/v_cos instead of sincos_f32
   int graph = 1;              // replay the restart cycle as a captured HIP graph
   int block[2] = {0, 0};      // far-field form per precision [FP64, FP32]:
-                              // 64, 128, 256 or kSymBlock;
+                              // 64, 128, 256, kSymBlock or (FP64) kSymRotBlock;
                               // 0 = timed on the batch at first use
   int ewald_pairs = 0;        // lattice solves: per-pair evaluation even on a grid
   int fft = 1;                // free-space solves filling a regular grid: FFT block-Toeplitz

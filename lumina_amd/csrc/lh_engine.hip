@@ -30,6 +30,8 @@ void launch_far_sym32(hipStream_t, bool, int, const FarItem*, const SolveDev*, c
                       const Src32*, double2*);
 void launch_far_sym64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src64*, double2*);
+void launch_far_symrot64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
+                         const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
                    const double2*, int, double*);
@@ -184,6 +186,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   if (max_rev > 200.0) tiling_.hw_sincos = 0;
   // the one-tensor-per-pair FP32 kernel uses the hardware sin/cos
   if (!tiling_.hw_sincos && tiling_.block[1] == kSymBlock) tiling_.block[1] = LH_TILE;
+  if (tiling_.block[1] == kSymRotBlock) tiling_.block[1] = LH_TILE;   // FP64 only
   const int S = (int)probs_.size();
   if (S == 0) throw Error(LH_INVALID_ARGUMENT, "empty batch");
 
@@ -399,10 +402,13 @@ void Engine::build_far_items(int p) {
   far_ewt_[p].clear();
   far_ewp_[p].clear();
   // before tuning, any block size serves (the far items are rebuilt then);
-  // kSymBlock selects the one-tensor-per-unordered-pair kernel for the
-  // free-space solves (items T >= S of 256 x 256 tiles), the others keep the
-  // row kernel with 256 targets per item
-  const bool sym = tiling_.block[p] == kSymBlock;
+  // kSymBlock and kSymRotBlock select the one-tensor-per-unordered-pair
+  // kernels for the free-space solves (items T >= S of 256 x 256 tiles; the
+  // LDS form splits off the items beyond the exclusion radius, the rotating
+  // form takes every item with the test), the others keep the row kernel
+  // with 256 targets per item
+  const bool rot = tiling_.block[p] == kSymRotBlock;
+  const bool sym = tiling_.block[p] == kSymBlock || rot;
   const int tb = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   for (size_t s = 0; s < sd_.size(); ++s) {
     const SolveDev& d = sd_[s];
@@ -416,7 +422,7 @@ void Engine::build_far_items(int p) {
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
-        else if (!(sym && free_space) || ti >= tj) {
+        else if (!(sym && free_space) || (ti >= tj && !rot)) {
           // items whose boxes lie beyond the exclusion radius (with a margin
           // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
           // decided per 256 x 256 tile pair, so that every block size sends
@@ -471,6 +477,7 @@ void Engine::autotune(int p) {
   float best_ms = 0.0f;
   std::vector<int> cands = {64, 128, 256};
   if (p == 0 || tiling_.hw_sincos) cands.push_back(kSymBlock);
+  if (p == 0) cands.push_back(kSymRotBlock);
   for (int cand : cands) {
     tiling_.block[p] = cand;
     build_far_items(p);
@@ -603,7 +610,8 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   const int nv = (int)vit_.size();
   const int p = fp32 ? 1 : 0;
   const int nfree = (int)far_free_[p].size(), nsub = (int)far_sub_[p].size();
-  const bool sym = tiling_.block[p] == kSymBlock;
+  const bool rot = tiling_.block[p] == kSymRotBlock;
+  const bool sym = tiling_.block[p] == kSymBlock || rot;
   const int blk = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   const VecItem* vi = d_vit_.as<VecItem>();
   const SolveDev* sd = d_solves_.as<SolveDev>();
@@ -644,7 +652,10 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
     if (timing_) LH_HIP(hipEventRecord(ev.a, st));
-    if (sym) {
+    if (rot) {
+      launch_far_symrot64(st, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
+                          d_src64_.as<Src64>(), d_part_.as<double2>());
+    } else if (sym) {
       launch_far_sym64(st, true, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
                        d_src64_.as<Src64>(), d_part_.as<double2>());
       launch_far_sym64(st, false, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd,

@@ -519,6 +519,188 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
 }
 
 // ---------------------------------------------------------------------------
+// Far field, one tensor per unordered pair (free-space solves), FP64, with
+// every source record rotating around the wave: no LDS staging and no barrier
+// inside the sub-block loop, which suits small batches whose few work items
+// leave one wave per SIMD (the autotuner's FP64 candidate kSymRotBlock;
+// k_far_sym64 below stages its sources in LDS and wins on large ones).
+//
+// G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
+// evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
+// item is a (target tile T, source tile S) pair with T >= S, 256 threads:
+// wave w holds targets T*256 + 64 w + lane and walks the four 64-source
+// sub-blocks of S. Each lane starts with its own source and the sources, with
+// their transposed accumulators, rotate one lane per step around the wave
+// (DPP wave_ror:1), so after 64 steps every target has met every source of the
+// sub-block and every accumulator is back in its lane. On the diagonal tile
+// (T == S) wave w takes sub-blocks b >= w only, and its own sub-block b == w
+// for steps 1..32 (step 32 one way only), so each unordered pair is counted
+// once. Transposed sums are combined over the four waves in LDS in a fixed
+// order: results are deterministic. Forward sums of T go to partial slice S,
+// transposed sums of S to slice T (on the diagonal both go to slice T).
+// Missing sources of a partial tile carry x = 0 and a copy of a real position:
+// their contribution is exactly 0 (a coincident pair is inside the exclusion
+// radius and skipped).
+// ---------------------------------------------------------------------------
+
+// DPP wave_ror:1 (0x13C) on both halves: lane l takes the value of lane l - 1
+// (lane 0 from 63)
+__device__ __forceinline__ double wave_ror1(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x13C, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One pair (target lane, source rotated into the lane): |d|^2 in FP64, the
+// exclusion decision on the FP32-rounded positions.
+template <typename R> struct SymPos;
+template <> struct SymPos<double> {
+  double x, y, z;
+  __device__ void set(const Src64& r) { x = r.x; y = r.y; z = r.z; }
+  __device__ void rotate() { x = wave_ror1(x); y = wave_ror1(y); z = wave_ror1(z); }
+};
+
+template <typename R>
+__device__ __forceinline__ void sym_coeffs(const SymPos<R>& t, const SymPos<R>& sp, float rskip2f,
+                                           R k, float krev, R inv_k, R c0, R& dx, R& dy, R& dz,
+                                           R& Ar, R& Ai, R& Br, R& Bi) {
+  dx = t.x - sp.x;
+  dy = t.y - sp.y;
+  dz = t.z - sp.z;
+  const float d2f = r2f_exact((float)t.x - (float)sp.x, (float)t.y - (float)sp.y,
+                              (float)t.z - (float)sp.z);
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  g_coeffs64(r2, k, inv_k, c0, Ar, Ai, Br, Bi, d2f <= rskip2f);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_far_sym(const FarItem* __restrict__ items,
+                                                 const SolveDev* __restrict__ solves,
+                                                 const GState* __restrict__ st, int which,
+                                                 const typename SrcOf<R>::type* __restrict__ src,
+                                                 double2* __restrict__ part) {
+  typedef typename SrcOf<R>::type SRC;
+  __shared__ R lds_t[4][4][64][6];   // [wave][sub-block][source lane][component]
+  const FarItem it = items[blockIdx.x];
+  if (!takes_part(st, it.s, which)) return;
+  const SolveDev S = solves[it.s];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = it.t0 + 64 * w + lane;
+  const bool tvalid = i < S.n;
+  const SRC* __restrict__ sp = src + S.src_off;
+  const SRC me = sp[tvalid ? i : 0];
+  SymPos<R> tp;
+  tp.set(me);
+  R vi[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : R(0);
+  const R k = (R)S.k, inv_k = (R)S.inv_k, c0 = (R)S.c0;
+  const float krev = (float)(S.k * 0.15915494309189535);
+  const float rskip2f = S.rskip2f;
+  const bool diag = it.t0 == it.c0;
+  R hi[6], lo[6], run[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = R(0);
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if constexpr (std::is_same<R, float>::value) two_sum(hi[q], lo[q], run[q]);
+      else hi[q] += run[q];
+      run[q] = R(0);
+    }
+  };
+
+  for (int b = 0; b < 4; ++b) {
+    const int jb = it.c0 + 64 * b;
+    R ta[6] = {0, 0, 0, 0, 0, 0};
+    if (jb >= S.n || (diag && b < w)) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) lds_t[w][b][lane][q] = R(0);
+      continue;
+    }
+    const int j = jb + lane;
+    const bool sv = j < S.n;
+    const SRC rs = sp[sv ? j : jb];
+    SymPos<R> spos;
+    spos.set(rs);
+    R vs[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vs[q] = sv ? rs.v[q] : R(0);
+
+    // one unordered pair per lane: forward into run, transposed into ta
+    auto step = [&](bool both) {
+      R dx, dy, dz, Ar, Ai, Br, Bi;
+      sym_coeffs<R>(tp, spos, rskip2f, k, krev, inv_k, c0, dx, dy, dz, Ar, Ai, Br, Bi);
+      apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+      if (both) apply<R>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
+    };
+    auto rotate = [&]() {
+      spos.rotate();
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        vs[q] = wave_ror1(vs[q]);
+        ta[q] = wave_ror1(ta[q]);
+      }
+    };
+    int home;   // rotation of the accumulators at the end
+    if (diag && b == w) {
+      for (int kk = 1; kk < 32; ++kk) {
+        rotate();
+        step(true);
+      }
+      rotate();
+      step(false);   // pairs (l, l - 32): each lane adds its own direction
+      fold();
+      home = 32;
+    } else {
+      for (int kk = 0; kk < 64; ++kk) {
+        step(true);
+        rotate();
+        if (kk == 31) fold();
+      }
+      fold();
+      home = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) lds_t[w][b][(lane - home) & 63][q] = ta[q];
+  }
+  __syncthreads();
+  // thread (w, lane): forward sum of target i; transposed sum of source
+  // c0 + 64 w + lane over the four waves
+  double tr[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    tr[q] = ((double)lds_t[0][w][lane][q] + (double)lds_t[1][w][lane][q]) +
+            ((double)lds_t[2][w][lane][q] + (double)lds_t[3][w][lane][q]);
+  const long long n3 = 3LL * S.n;
+  if (diag) {
+    if (tvalid) {
+      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o[c] = make_double2(((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c],
+                            ((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]);
+    }
+  } else {
+    if (tvalid) {
+      double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o[c] = make_double2((double)hi[2 * c] + (double)lo[2 * c],
+                            (double)hi[2 * c + 1] + (double)lo[2 * c + 1]);
+    }
+    const int jj = it.c0 + 64 * w + lane;
+    if (jj < S.n) {
+      double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * jj;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[c] = make_double2(tr[2 * c], tr[2 * c + 1]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 far field, one tensor per unordered pair, sources staged in LDS.
 //
 // The scheme of k_far_sym32 in FP64: items T >= S, wave w holds targets
@@ -868,6 +1050,15 @@ void launch_far_sym32(hipStream_t st, bool near, int nitems, const FarItem* item
   else
     hipLaunchKernelGGL((k_far_sym32<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
                        which, src, part);
+}
+
+// The rotating form (k_far_sym<double>), every item T >= S with the
+// exclusion test.
+void launch_far_symrot64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
+                         const GState* gs, int which, const Src64* src, double2* part) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL((k_far_sym<double>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
+                     which, src, part);
 }
 
 // near = false: items without excluded pairs (k_far_sym64<false>).

@@ -273,14 +273,14 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             if precision == MIXED:
                 assert kt["far32_block"] == 512
         else:
-            assert kt["far64_block"] in (64, 128, 256, 512)
+            assert kt["far64_block"] in (64, 128, 256, 512, 1024)
         runs[name] = (cs, [i["iterations"] for i in infos], p,
                       (kt["far64_block"], kt["far32_block"]))
         print(name, runs[name][1], kt["far64_block"], kt["far32_block"], kt["tune_ms"])
     cs0, it0, p0, _ = runs["b256"]
     for name, (cs, it, p, (f64, f32)) in runs.items():
         # the one-tensor-per-pair kernels (block 512) sum in another order
-        exact = f64 != 512 and not (precision == MIXED and f32 == 512)
+        exact = f64 not in (512, 1024) and not (precision == MIXED and f32 == 512)
         if exact:
             assert it == it0, name
             assert np.array_equal(cs, cs0), name
@@ -295,11 +295,13 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
                     assert np.all(np.abs(cs[q] - cs0[q]) <= 1e-9 * np.abs(cs0[q][0])), (name, q)
 
 
+@pytest.mark.parametrize("form", ["sym", "1024"])
 @pytest.mark.parametrize("n", [1, 2, 65, 257, 515, 700])
-def test_matvec_fp64_symmetric_kernel(backend, n, monkeypatch):
-    """The FP64 one-tensor-per-unordered-pair kernel against the oracle (FP64
-    bar), and a solve through it against the row kernel's."""
-    monkeypatch.setenv("LUMINA_FAR_BLOCK", "sym")
+def test_matvec_fp64_symmetric_kernel(backend, n, form, monkeypatch):
+    """The FP64 one-tensor-per-unordered-pair kernels (LDS-staged sources,
+    "sym"; sources rotating around the wave, "1024") against the oracle (FP64
+    bar), and a solve through each against the row kernel's."""
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", form)
     if n == 515:
         pos, alpha, k, cell = sphere_problem()
     else:
