@@ -246,9 +246,12 @@ def run_batch(args):
     if not args.no_fp64_plain:
         sub = probs[args.wavelengths // 2::args.wavelengths][: args.plain_count]
         t_pl, ev_pl = 0.0, 0.0
-        # the far-field block size the batch's tuning chose for FP64, so the
-        # sessions below do not each time the candidates inside the solve
-        os.environ.setdefault("LUMINA_FAR_BLOCK", str(blocks[0]))
+        # the far-field form the batch's tuning chose for FP64, so the sessions
+        # below do not each time the candidates inside the solve ("sym" selects
+        # the unordered-pair form for FP64 as well: "512" alone keeps the FP64
+        # row kernel)
+        fp64_form = {512: "sym"}.get(blocks[0], str(blocks[0]))
+        os.environ.setdefault("LUMINA_FAR_BLOCK", fp64_form)
         for p in sub:
             s_ = PairSumSession(be, p["positions"], p["polarisability"], p["k"], True,
                                 p["cell_size"], precision=FP64)
@@ -261,7 +264,7 @@ def run_batch(args):
             ev_pl += info["pair_evals"]
             s_.close()
         plain = {"pair_evals_per_s": ev_pl / t_pl, "solves_per_s": len(sub) / t_pl,
-                 "solves": len(sub),
+                 "solves": len(sub), "far64_form": os.environ.get("LUMINA_FAR_BLOCK"),
                  "sample": f"{len(sub)} problems (each particle at the middle wavelength), "
                            f"one FP64 session and solve each, tol {args.tol:g}"}
         log(f"[bench] plain FP64: {plain['pair_evals_per_s']:.3e} pair-evals/s")
