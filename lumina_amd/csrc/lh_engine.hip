@@ -48,7 +48,7 @@ void launch_cross(hipStream_t, int, const VecItem*, const SolveDev*, const doubl
                   const double2*, double*);
 void launch_copy_slot(hipStream_t, int, const VecItem*, const SolveDev*, double2*, int, int);
 void launch_zero(hipStream_t, int*);
-void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, const double2*, int);
+void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, const double2*, int, int*);
 void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const double*, int, int*);
 void launch_backsub(hipStream_t, int, GState*);
 void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*);
@@ -707,10 +707,10 @@ void Engine::arnoldi_step(int j, bool fp32) {
   matvec(0, j, LH_SLOT_W, 0, fp32);
   for (int pass = 0; pass < 2; ++pass) {
     launch_dots(st, nv, vi, sd, gs, vec, j, d_dot_.as<double2>());
-    launch_fin_dots(st, S, sd, gs, j, d_dot_.as<double2>(), pass);
+    // pass 0 also clears the counter k_fin_step counts into
+    launch_fin_dots(st, S, sd, gs, j, d_dot_.as<double2>(), pass, d_counter_.as<int>());
     launch_update(st, nv, vi, sd, gs, vec, j, d_norm_.as<double>());
   }
-  launch_zero(st, d_counter_.as<int>());
   launch_fin_step(st, S, sd, gs, j, d_norm_.as<double>(), fp32 ? 1 : 0, d_counter_.as<int>());
   launch_scale(st, nv, vi, sd, gs, 2, vec, LH_SLOT_W, -1);
 }

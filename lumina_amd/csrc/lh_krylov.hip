@@ -295,11 +295,14 @@ __global__ __launch_bounds__(64) void k_zero(int* __restrict__ counter) {
 // hp_i = sum over items of the dot partials; hc accumulates over passes.
 // Four waves per solve, wave w takes i = w, w + 4, ...; its lanes stride
 // over the items and combine with a fixed butterfly (deterministic).
+// Pass 0 also clears the step counter that k_fin_step of the same Arnoldi
+// step counts into (block 0, before any block leaves).
 __global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ solves,
                                                  GState* __restrict__ st, int j,
                                                  const double2* __restrict__ dot_part,
-                                                 int pass) {
+                                                 int pass, int* __restrict__ counter) {
   const int s = blockIdx.x;
+  if (pass == 0 && s == 0 && threadIdx.x == 0) *counter = 0;
   GState& g = st[s];
   if (!g.inner) return;
   const SolveDev S = solves[s];
@@ -536,8 +539,8 @@ void launch_zero(hipStream_t st, int* counter) {
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, counter);
 }
 void launch_fin_dots(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
-                     const double2* dp, int pass) {
-  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, j, dp, pass);
+                     const double2* dp, int pass, int* counter) {
+  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, j, dp, pass, counter);
 }
 void launch_fin_step(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
                      const double* np, int fp32, int* counter) {
