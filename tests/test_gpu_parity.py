@@ -242,14 +242,15 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
     host-issued cycle, and every row-kernel block size, give bit-identical
     moments and iteration counts, because each target's sum over a
     256-source tile is formed in the same order. The one-tensor-per-pair
-    kernels (block 512) sum in another order: their results agree to the
-    solve tolerance."""
+    kernels (block 512; FP64 with rotating sources, block 1024) sum in
+    another order: their results agree to the solve tolerance."""
     from lumina_amd.solver import Batch
     probs = _batch_probs()
     runs = {}
     opts = [("auto", {}), ("nograph", {"LUMINA_GRAPH": "0", "LUMINA_FAR_BLOCK": "256"}),
             ("b64", {"LUMINA_FAR_BLOCK": "64"}), ("b128", {"LUMINA_FAR_BLOCK": "128"}),
-            ("b256", {"LUMINA_FAR_BLOCK": "256"}), ("sym", {"LUMINA_FAR_BLOCK": "512"})]
+            ("b256", {"LUMINA_FAR_BLOCK": "256"}), ("sym", {"LUMINA_FAR_BLOCK": "512"}),
+            ("rot", {"LUMINA_FAR_BLOCK": "1024"})]
     for name, env in opts:
         for key in ("LUMINA_GRAPH", "LUMINA_FAR_BLOCK"):
             monkeypatch.delenv(key, raising=False)
@@ -272,6 +273,10 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             assert kt["far64_block"] == 256
             if precision == MIXED:
                 assert kt["far32_block"] == 512
+        elif name == "rot":
+            assert kt["far64_block"] == 1024
+            if precision == MIXED:
+                assert kt["far32_block"] == 256
         else:
             assert kt["far64_block"] in (64, 128, 256, 512, 1024)
         runs[name] = (cs, [i["iterations"] for i in infos], p,
