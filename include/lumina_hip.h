@@ -82,6 +82,18 @@ int lh_session_create(lh_backend* backend, size_t n, lh_session** out);
 int lh_session_upload_matrix(lh_session* session, const lh_c64* matrix, size_t n);
 /* MatvecSession::matvec (backend.rs:54; gpu.rs:428-488): y = A x, length n. */
 int lh_session_matvec(lh_session* session, const lh_c64* x, lh_c64* y, size_t n);
+/* ComputeBackend::assemble_and_create_session (backend.rs:109-117; the wgpu
+ * form gpu.rs:508-607 with shaders/matrix_assemble.wgsl), called by
+ * CdaSolver for FCD off, no lattice, no substrate, off-grid dipoles
+ * (cda/mod.rs:298-311): a session of n = 3 n_dipoles unknowns holding
+ * A = blockdiag(diagonal_blocks) - G, G the point tensor (greens.rs:34-71)
+ * of r_i - r_j, assembled on the device in FP64 (the wgpu form assembles
+ * in FP32). diagonal_blocks: [n_dipoles][9] row-major alpha^-1
+ * (compute_diagonal_blocks, assembly.rs:355-363). A pair closer than
+ * 1e-10 nm gets a zero block, as in the wgpu shader. The reference's
+ * None for n_dipoles == 0 is LH_INVALID_ARGUMENT here. */
+int lh_session_assemble(lh_backend* backend, const double* positions, size_t n_dipoles, double k,
+                        const lh_c64* diagonal_blocks, lh_session** out);
 void lh_session_destroy(lh_session* session);
 
 /* ---- on-the-fly pair-sum session --------------------------------------- */

@@ -28,7 +28,8 @@ LH_PREC_MIXED = 1
 # Every symbol include/lumina_hip.h declares.
 EXPORTS = [
     "lh_last_error", "lh_backend_create", "lh_backend_destroy", "lh_backend_device_info",
-    "lh_session_create", "lh_session_upload_matrix", "lh_session_matvec", "lh_session_destroy",
+    "lh_session_create", "lh_session_upload_matrix", "lh_session_matvec", "lh_session_assemble",
+    "lh_session_destroy",
     "lh_pairsum_create", "lh_pairsum_set_wavelength", "lh_pairsum_matvec", "lh_pairsum_solve",
     "lh_pairsum_incident", "lh_pairsum_cross_sections", "lh_pairsum_compute_cross_sections",
     "lh_pairsum_adjoint_grad", "lh_pairsum_field_at_points", "lh_pairsum_far_field",
@@ -146,6 +147,7 @@ def load():
     lib.lh_session_create.argtypes = [P, C.c_size_t, PP]
     lib.lh_session_upload_matrix.argtypes = [P, P, C.c_size_t]
     lib.lh_session_matvec.argtypes = [P, P, P, C.c_size_t]
+    lib.lh_session_assemble.argtypes = [P, D, C.c_size_t, C.c_double, P, PP]
     lib.lh_session_destroy.argtypes = [P]
     lib.lh_session_destroy.restype = None
     lib.lh_pairsum_create.argtypes = [P, C.POINTER(PairsumSpec), PP]

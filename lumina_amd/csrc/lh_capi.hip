@@ -252,6 +252,8 @@ struct lh_batch {
 
 namespace lh {
 void launch_dense_matvec(hipStream_t st, int n, const double2* A, const double2* x, double2* y);
+void launch_assemble(hipStream_t st, int n, const double* pos, double k, const double2* diag,
+                     double2* A);
 }
 
 extern "C" {
@@ -352,6 +354,43 @@ int lh_session_matvec(lh_session* s, const lh_c64* x, lh_c64* y, size_t n) {
     LH_HIP(hipGetLastError());
     LH_HIP(hipMemcpyAsync(y, s->y.p, n * sizeof(lh_c64), hipMemcpyDeviceToHost, s->stream));
     LH_HIP(hipStreamSynchronize(s->stream));
+    return (int)LH_OK;
+  });
+}
+
+int lh_session_assemble(lh_backend* b, const double* positions, size_t n_dipoles, double k,
+                        const lh_c64* diagonal_blocks, lh_session** out) {
+  return guarded([&]() {
+    if (!b || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (n_dipoles == 0) return fail(LH_INVALID_ARGUMENT, "No dipoles provided");
+    if (!positions || !diagonal_blocks) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(std::isfinite(k) && k > 0.0))
+      return fail(LH_INVALID_ARGUMENT, "k must be positive and finite");
+    // one grid row per block row i (gridDim.y); the dense matrix of 65,535
+    // dipoles is 619 GB, beyond any device memory
+    if (n_dipoles > 65535)
+      return fail(LH_INVALID_ARGUMENT, "dense assembly is limited to 65535 dipoles");
+    for (size_t q = 0; q < 3 * n_dipoles; ++q)
+      if (!std::isfinite(positions[q])) return fail(LH_INVALID_ARGUMENT, "non-finite dipole position");
+    for (size_t q = 0; q < 9 * n_dipoles; ++q)
+      if (!std::isfinite(diagonal_blocks[q].re) || !std::isfinite(diagonal_blocks[q].im))
+        return fail(LH_INVALID_ARGUMENT, "non-finite diagonal block");
+    lh_session* raw = nullptr;
+    const int st = lh_session_create(b, 3 * n_dipoles, &raw);
+    if (st != LH_OK) return st;
+    std::unique_ptr<lh_session, void (*)(lh_session*)> s(raw, lh_session_destroy);
+    lh::DevMem pos, diag;
+    pos.alloc(3 * n_dipoles * sizeof(double));
+    diag.alloc(9 * n_dipoles * sizeof(lh_c64));
+    LH_HIP(hipMemcpyAsync(pos.p, positions, pos.bytes, hipMemcpyHostToDevice, s->stream));
+    LH_HIP(hipMemcpyAsync(diag.p, diagonal_blocks, diag.bytes, hipMemcpyHostToDevice, s->stream));
+    lh::launch_assemble(s->stream, (int)n_dipoles, pos.as<double>(), k, diag.as<double2>(),
+                        s->A.as<double2>());
+    LH_HIP(hipGetLastError());
+    LH_HIP(hipStreamSynchronize(s->stream));
+    s->uploaded = true;
+    *out = s.release();
     return (int)LH_OK;
   });
 }

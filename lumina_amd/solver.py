@@ -177,12 +177,14 @@ def _spec(positions, polarisability, k, use_fcd, cell_size, substrate, precision
 class MatvecSession:
     """Stored-matrix session (MatvecSession, backend.rs:47-55)."""
 
-    def __init__(self, backend: "HipBackend", n: int):
+    def __init__(self, backend: "HipBackend", n: int, _handle=None):
         self._lib = backend._lib
         self.n = int(n)
-        h = C.c_void_p()
-        check(self._lib.lh_session_create(backend._h, self.n, C.byref(h)))
-        self._h = h
+        if _handle is None:
+            h = C.c_void_p()
+            check(self._lib.lh_session_create(backend._h, self.n, C.byref(h)))
+            _handle = h
+        self._h = _handle
 
     def upload_matrix(self, matrix) -> None:
         m = _c128(matrix)
@@ -341,6 +343,20 @@ class HipBackend:
 
     def create_pairsum_session(self, *a, **kw) -> PairSumSession:
         return PairSumSession(self, *a, **kw)
+
+    def assemble_and_create_session(self, positions, k: float, diagonal_blocks) -> MatvecSession:
+        """A stored-matrix session holding A = blockdiag(diagonal_blocks) - G,
+        assembled on the device (assemble_and_create_session,
+        backend.rs:109-117; gpu.rs:508-607). ``diagonal_blocks``: (N, 3, 3)
+        or (N, 9) alpha^-1 blocks, row-major."""
+        pos = _f64(positions).reshape(-1, 3)
+        blocks = _c128(diagonal_blocks).reshape(-1, 9)
+        if blocks.shape[0] != pos.shape[0]:
+            raise ValueError("one diagonal block per dipole")
+        h = C.c_void_p()
+        check(self._lib.lh_session_assemble(self._h, dptr(pos), pos.shape[0], float(k),
+                                            cptr(blocks), C.byref(h)))
+        return MatvecSession(self, 3 * pos.shape[0], _handle=h)
 
     def close(self):
         if getattr(self, "_h", None):
