@@ -60,16 +60,15 @@ __device__ __forceinline__ void g_coeffs64(double r2, double k, double inv_k, do
   double s, c;
   sincos_f64(k * r, s, c);
   const double u = inv_r * inv_k;  // 1 / (k r)
-  const double u2 = u * u;
   const double pf = c0 * inv_r;
-  const double pr = pf * c, pi = pf * s;
-  const double ar = 1.0 - u2, ai = u;
-  const double br = 3.0 * u2 - 1.0, bi = -3.0 * u;
-  const double ir2 = inv_r * inv_r;
-  Ar = pr * ar - pi * ai;
-  Ai = pr * ai + pi * ar;
-  Br = (pr * br - pi * bi) * ir2;
-  Bi = (pr * bi + pi * br) * ir2;
+  const double pr = pf * c, pi = pf * s;  // g = pf e^{ikr}
+  const double ar = fma(-u, u, 1.0);      // a = (1 - u^2) + iu
+  Ar = pr * ar - pi * u;
+  Ai = pr * u + pi * ar;
+  // b = (3u^2 - 1) - 3iu = 2 - 3a, so B = g b / r^2 = (g - 1.5 A)(2 / r^2)
+  const double h = 2.0 * (inv_r * inv_r);
+  Br = fma(-1.5, Ar, pr) * h;
+  Bi = fma(-1.5, Ai, pi) * h;
 }
 
 // The six independent entries (xx yy zz xy xz yz) of the point tensor.

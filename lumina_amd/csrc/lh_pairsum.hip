@@ -167,18 +167,15 @@ __device__ __forceinline__ void coeffs32(float r2, bool skip, float k, float kre
   }
   const float u = inv_r * inv_k;       // 1 / (k r)
   const float pf = c0 * inv_r;         // k^2 / (4 pi r)
-  const float pr = pf * c, pi = pf * s;
-  const float ar = fmaf(-u, u, 1.0f), ai = u;
-  // b / r^2 with b = (3u^2 - 1) - 3iu, from t2 = 1/r^2 and v3 = 3u/r^2:
-  // (u v3 - t2) - i v3
-  const float t2 = inv_r * inv_r;
-  const float v3 = 3.0f * (t2 * u);
-  const float bi = -v3;
-  const float br = fmaf(u, v3, -t2);
-  Ar = pr * ar - pi * ai;
-  Ai = pr * ai + pi * ar;
-  Br = pr * br - pi * bi;
-  Bi = pr * bi + pi * br;
+  const float pr = pf * c, pi = pf * s;  // g = pf e^{ikr}
+  const float ar = fmaf(-u, u, 1.0f);  // a = (1 - u^2) + iu
+  Ar = pr * ar - pi * u;
+  Ai = pr * u + pi * ar;
+  // b = (3u^2 - 1) - 3iu = 2 - 3a, so B = g b / r^2 = (g - 1.5 A)(2 / r^2):
+  // two FMA and two products from A instead of forming b
+  const float h = 2.0f * (inv_r * inv_r);
+  Br = fmaf(-1.5f, Ar, pr) * h;
+  Bi = fmaf(-1.5f, Ai, pi) * h;
 }
 
 // out += (A I + B d d^T) v, as fused multiply-add chains into the
