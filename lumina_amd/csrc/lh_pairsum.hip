@@ -871,39 +871,11 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
     const double2* __restrict__ ainv, double2* __restrict__ vec, int in_slot, int out_slot,
     const double2* __restrict__ part, int mode, double* __restrict__ norm_part) {
   __shared__ double lds[4];
-  __shared__ double2 psum[3 * LH_VB];   // the item's partial-slice totals, element order
   const VecItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
   const int i = it.t0 + threadIdx.x;
   double nrm = 0.0;
-  {
-    // the far partial slices, summed over the slices in slice order with the
-    // item's 3 LH_VB elements spread over the block (thread t: elements t,
-    // t + LH_VB, t + 2 LH_VB), so every load is a contiguous block-wide read;
-    // then handed to one dipole per thread through LDS
-    const long long n3 = 3LL * S.n;
-    const long long e0 = 3LL * it.t0;
-    const int nslice = S.fft ? 1 : S.nt;   // the FFT matvec fills slice 0
-    double2 acc[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) acc[q] = make_double2(0.0, 0.0);
-    for (int ch = 0; ch < nslice; ++ch) {
-      const double2* p = part + S.part_off + (long long)ch * n3 + e0;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const long long e = threadIdx.x + q * LH_VB;
-        if (e0 + e < n3) {
-          const double2 v = p[e];
-          acc[q].x += v.x;
-          acc[q].y += v.y;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) psum[threadIdx.x + q * LH_VB] = acc[q];
-  }
-  __syncthreads();
   if (i < S.n) {
     const long long n3 = 3LL * S.n;
     const double2* xin = vec + S.vec_off + (long long)in_slot * n3;
@@ -928,11 +900,21 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
       yr[a] = sr;
       yi[a] = si;
     }
+    // the far partial slices, one per source tile (the FFT matvec fills
+    // slice 0), each thread reading its dipole's 48 contiguous bytes per
+    // slice. Summing them in element order across the block instead (fully
+    // contiguous block-wide loads, totals passed through LDS) measured within
+    // 1% on the 1,024-solve batch and 1.7x slower on a single 38,637-dipole
+    // solve, whose 151 blocks leave CUs idle.
+    const int nslice = S.fft ? 1 : S.nt;
+    for (int ch = 0; ch < nslice; ++ch) {
+      const double2* p = part + S.part_off + (long long)ch * n3 + 3LL * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double2 q = psum[3 * threadIdx.x + c];
-      yr[c] -= q.x;
-      yi[c] -= q.y;
+      for (int c = 0; c < 3; ++c) {
+        const double2 q = p[c];
+        yr[c] -= q.x;
+        yi[c] -= q.y;
+      }
     }
     const int kw = S.fft == 2 ? 0 : S.kw;   // fft == 2: no near list
     for (int kk = 0; kk < kw; ++kk) {
