@@ -188,6 +188,8 @@ lh::Tiling tiling_from_env() {
   const char* ff = std::getenv("LUMINA_FFT");
   if (ff && std::string(ff) == "0") t.fft = 0;
   if (ff && std::string(ff) == "force") t.fft = 2;
+  const char* cg = std::getenv("LUMINA_CGS2");
+  if (cg && std::string(cg) == "1") t.cgs2_fp32 = 1;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
   if (bl && std::string(bl) == "sym") t.block[0] = t.block[1] = lh::kSymBlock;
   if (bl) {

@@ -49,6 +49,8 @@ struct Tiling {
   int fft = 1;                // free-space solves filling a regular grid: FFT block-Toeplitz
                               // matvec instead of the pair sum; 1 = from kFftMinDipoles
                               // dipoles, 2 = from 2, 0 = never
+  int cgs2_fp32 = 0;          // Arnoldi steps with an FP32 matvec: classical Gram-Schmidt
+                              // with reorthogonalisation (1) or one classical pass (0)
 };
 
 // Device arrays of a sweep's per-wavelength setup (lh_sweep.hip).

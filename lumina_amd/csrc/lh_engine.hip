@@ -705,7 +705,16 @@ void Engine::arnoldi_step(int j, bool fp32) {
   GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
   matvec(0, j, LH_SLOT_W, 0, fp32);
-  for (int pass = 0; pass < 2; ++pass) {
+  // FP64 steps: classical Gram-Schmidt with one reorthogonalisation pass
+  // (CGS2), orthogonal to working precision like the reference's MGS
+  // (iterative.rs:92-96). Steps with an FP32 matvec take one classical pass
+  // (FP64 arithmetic): the basis only has to be as good as the FP32 products
+  // it spans, and every cycle restarts from an FP64 residual. On the
+  // 1,024-solve batch this left the iteration counts unchanged (623.11 mean,
+  // 607 converged, C_ext within 8.9e-8 of FP64) and the helix's forward and
+  // adjoint counts identical, at half the Gram-Schmidt traffic.
+  const int passes = (fp32 && !tiling_.cgs2_fp32) ? 1 : 2;
+  for (int pass = 0; pass < passes; ++pass) {
     launch_dots(st, nv, vi, sd, gs, vec, j, d_dot_.as<double2>());
     // pass 0 also clears the counter k_fin_step counts into
     launch_fin_dots(st, S, sd, gs, j, d_dot_.as<double2>(), pass, d_counter_.as<int>());
