@@ -310,9 +310,13 @@ void lh_batch_destroy(lh_batch* batch);
 
 /* Instrumentation (no reference counterpart): when enabled, the far-field
  * pair-sum launches of lh_batch_run are bracketed by HIP events on the
- * batch's stream and their device time is accumulated. pairs_* count the
- * Green's pair evaluations those launches performed (N(N-1), plus N^2 image
- * pairs, per participating solve). */
+ * batch's stream (event nodes of the captured restart-cycle graphs, which
+ * keep running as graphs) and their device time is accumulated. pairs_*
+ * count the Green's pair evaluations those launches performed (N(N-1), plus
+ * N^2 image pairs, per participating solve). host_round_trips = graph
+ * launches (or host-issued cycles) of the last lh_batch_run, each followed
+ * by one read of the running-solve counter; cycles_issued = restart cycles
+ * they issued (a graph launch replays several). */
 typedef struct {
   double far64_ms, far32_ms;
   uint64_t far64_launches, far32_launches;
@@ -321,6 +325,9 @@ typedef struct {
   int32_t far64_block;    /* far-field targets per work item in use (FP64, FP32) */
   int32_t far32_block;
   double tune_ms;         /* time spent choosing them (0 when fixed) */
+  int32_t host_round_trips;
+  int32_t reserved;
+  uint64_t cycles_issued;
 } lh_kernel_times;
 int lh_batch_set_timing(lh_batch* batch, int enable);
 int lh_batch_kernel_times(lh_batch* batch, lh_kernel_times* out);

@@ -108,7 +108,9 @@ class KernelTimes(C.Structure):
                 ("far64_launches", C.c_uint64), ("far32_launches", C.c_uint64),
                 ("far64_pairs", C.c_double), ("far32_pairs", C.c_double),
                 ("run_ms", C.c_double), ("far64_block", C.c_int32),
-                ("far32_block", C.c_int32), ("tune_ms", C.c_double)]
+                ("far32_block", C.c_int32), ("tune_ms", C.c_double),
+                ("host_round_trips", C.c_int32), ("reserved", C.c_int32),
+                ("cycles_issued", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

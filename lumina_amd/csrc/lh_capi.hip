@@ -188,6 +188,9 @@ lh::Tiling tiling_from_env() {
   const char* ff = std::getenv("LUMINA_FFT");
   if (ff && std::string(ff) == "0") t.fft = 0;
   if (ff && std::string(ff) == "force") t.fft = 2;
+  const char* gc = std::getenv("LUMINA_GRAPH_CYCLES");
+  if (gc && std::string(gc) == "all") t.graph_cycles = -1;
+  else if (gc && std::atoi(gc) > 0) t.graph_cycles = std::atoi(gc);
   const char* cg = std::getenv("LUMINA_CGS2");
   if (cg && std::string(cg) == "1") t.cgs2_fp32 = 1;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
@@ -400,6 +403,11 @@ int lh_session_assemble(lh_backend* b, const double* positions, size_t n_dipoles
 void lh_session_destroy(lh_session* s) {
   if (!s) return;
   (void)hipSetDevice(s->b->device);
+  // queued work first, then the memory it uses, then the stream
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  s->A.release();
+  s->x.release();
+  s->y.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -775,6 +783,8 @@ int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
     out->far64_block = bt->eng->block(0);
     out->far32_block = bt->eng->block(1);
     out->tune_ms = bt->eng->tune_ms;
+    out->host_round_trips = bt->eng->last_launches;
+    out->cycles_issued = (uint64_t)bt->eng->last_cycles_issued;
     return (int)LH_OK;
   });
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -29,6 +30,7 @@ struct DevMem {
   DevMem& operator=(const DevMem&) = delete;
   ~DevMem();
   void alloc(size_t b);
+  void release();
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
@@ -51,7 +53,12 @@ struct Tiling {
                               // dipoles, 2 = from 2, 0 = never
   int cgs2_fp32 = 0;          // Arnoldi steps with an FP32 matvec: classical Gram-Schmidt
                               // with reorthogonalisation (1) or one classical pass (0)
+  int graph_cycles = 0;       // restart cycles per graph launch: 0 = 1, 2, 4, ... up to
+                              // kAutoGraphCycles; N > 0 = N; -1 = enough cycles to reach
+                              // max_iter in one launch (at most kMaxGraphCycles)
 };
+constexpr int kAutoGraphCycles = 8;
+constexpr int kMaxGraphCycles = 64;
 
 // Device arrays of a sweep's per-wavelength setup (lh_sweep.hip).
 struct SweepDevArgs {
@@ -123,13 +130,19 @@ class Engine {
   void sweep_setup(const SweepSetup& su, const lh_incident& inc);
   hipStream_t stream() const { return stream_; }
 
-  // Far-field launch timing (HIP events on the engine's stream).
-  void set_timing(bool on) { timing_ = on; }
+  // Far-field launch timing: HIP events around every far-field launch on
+  // the engine's stream, recorded into the captured graphs as event nodes
+  // when the cycle is replayed from a graph.
+  void set_timing(bool on);
   // Sums over the far launches since the last solve() started:
   // {far64_ms, far32_ms, far64_launches, far32_launches}.
   void far_times(double out[4]);
   double last_run_ms = 0.0;
   double tune_ms = 0.0;
+  // host round trips (graph launches or host-issued cycles, each followed by
+  // one counter read) and restart cycles issued by the last solve()
+  int last_launches = 0;
+  long long last_cycles_issued = 0;
   int block(int fp32) const { return tiling_.block[fp32]; }
   int fft_solves() const { return (int)fft_.size(); }
 
@@ -138,13 +151,26 @@ class Engine {
     hipEvent_t a, b;
     bool fp32;
   };
-  std::vector<EvPair> evs_;
+  std::vector<EvPair> evs_;                 // host-issued launches' events not yet read
+  std::vector<EvPair>* cap_evs_ = nullptr;  // events of the graph being captured
+  double far_ms_[2] = {0.0, 0.0};
+  long long far_n_[2] = {0, 0};
   bool timing_ = false;
   void clear_events();
+  void record(hipEvent_t e);
+  // adds the elapsed times of evs to the far-field sums (destroy: and frees them)
+  void harvest(std::vector<EvPair>& evs, bool destroy);
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
   void arnoldi_step(int j, bool fp32);
   void enqueue_cycle(bool mixed, bool early_exit);
-  hipGraphExec_t cycle_graph_[2] = {nullptr, nullptr};   // [fp64, mixed]
+  // K restart cycles captured once as a HIP graph and replayed, keyed by
+  // (K, timing events, precision)
+  struct CycleGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<EvPair> evs;
+  };
+  std::map<int, CycleGraph> graphs_;
+  CycleGraph& cycle_graph(bool mixed, int cycles);
   void drop_graphs();
   // far work items of precision p for the block size tiling_.block[p]
   void build_far_items(int p);

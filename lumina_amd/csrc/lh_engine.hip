@@ -100,8 +100,12 @@ void check_hip(hipError_t e, const char* what) {
   throw Error(st, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-DevMem::~DevMem() {
+DevMem::~DevMem() { release(); }
+
+void DevMem::release() {
   if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
 }
 
 void DevMem::alloc(size_t b) {
@@ -132,10 +136,15 @@ static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
     LH_HIP(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// Teardown order: every kernel queued on the engine's stream finishes first;
+// then the graph executables and timing events that refer to the engine's
+// memory, then the hipFFT plans (and their buffers), then, as members, the
+// device memory. The owner destroys the stream last.
 Engine::~Engine() {
+  (void)hipStreamSynchronize(stream_);
   clear_events();
-  for (auto& g : cycle_graph_)
-    if (g) (void)hipGraphExecDestroy(g);
+  drop_graphs();
+  fft_.clear();
 }
 
 namespace {
@@ -164,26 +173,10 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   geoms_ = std::move(geoms);
   probs_ = std::move(probs);
   tiling_ = t;
-  // The hardware v_sin/v_cos take the phase in revolutions and are valid
-  // for |x| <= 256 revolutions; the largest far-field phase of a solve is
-  // k times its geometry's bounding-box diagonal (its image pairs included:
-  // the image of a source is at most twice as far).
+  // hardware sin/cos in the FP32 far field only within their phase range
   double max_rev = 0.0;
-  for (const auto& p : probs_) {
-    const Geometry& g = geoms_[p.geom];
-    if (g.n == 0) continue;
-    double lo[3] = {g.x[0], g.y[0], g.z[0]}, hi[3] = {g.x[0], g.y[0], g.z[0]};
-    for (int i = 0; i < g.n; ++i) {
-      lo[0] = std::min(lo[0], g.x[i]); hi[0] = std::max(hi[0], g.x[i]);
-      lo[1] = std::min(lo[1], g.y[i]); hi[1] = std::max(hi[1], g.y[i]);
-      lo[2] = std::min(lo[2], g.z[i]); hi[2] = std::max(hi[2], g.z[i]);
-    }
-    double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
-                            (hi[2] - lo[2]) * (hi[2] - lo[2]));
-    if (p.has_sub) diag = 2.0 * diag + 2.0 * std::fabs(p.z_interface) + 2.0 * std::max(std::fabs(lo[2]), std::fabs(hi[2]));
-    max_rev = std::max(max_rev, p.k * diag / (2.0 * M_PI));
-  }
-  if (max_rev > 200.0) tiling_.hw_sincos = 0;
+  for (const auto& p : probs_) max_rev = std::max(max_rev, max_phase_revolutions(geoms_[p.geom], p));
+  if (max_rev > kMaxHwRevolutions) tiling_.hw_sincos = 0;
   // the one-tensor-per-pair FP32 kernel uses the hardware sin/cos
   if (!tiling_.hw_sincos && tiling_.block[1] == kSymBlock) tiling_.block[1] = LH_TILE;
   if (tiling_.block[1] == kSymRotBlock) tiling_.block[1] = LH_TILE;   // FP64 only
@@ -449,11 +442,14 @@ void Engine::build_far_items(int p) {
 }
 
 void Engine::drop_graphs() {
-  for (auto& g : cycle_graph_)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
+  for (auto& kv : graphs_) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto& e : kv.second.evs) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
     }
+  }
+  graphs_.clear();
 }
 
 bool Engine::has_pair_work() const {
@@ -522,6 +518,7 @@ void Engine::set_problem(int s, const Problem& p) {
   q.k = p.k;
   q.ainv = p.ainv;
   q.delta = p.delta;
+  const bool hw_off = tiling_.hw_sincos && max_phase_revolutions(g, q) > kMaxHwRevolutions;
   SolveDev& d = sd_[s];
   d.k = p.k;
   d.inv_k = 1.0 / p.k;
@@ -545,6 +542,15 @@ void Engine::set_problem(int s, const Problem& p) {
                         q.ainv.size() * sizeof(lh_c64), hipMemcpyHostToDevice, stream_));
   LH_HIP(hipMemcpyAsync(d_solves_.as<SolveDev>() + s, &d, sizeof(SolveDev),
                         hipMemcpyHostToDevice, stream_));
+  if (hw_off) {
+    // the new k takes the FP32 far field beyond the hardware sin/cos range:
+    // software sincos_f32 from now on, and off the unordered-pair FP32 form,
+    // which uses the hardware functions (rebuilding the items drops the
+    // captured graphs, which hold the old launch arguments)
+    tiling_.hw_sincos = 0;
+    if (tiling_.block[1] == kSymBlock) tiling_.block[1] = LH_TILE;
+    build_far_items(1);
+  }
   compute_near_tensors();
   LH_HIP(hipStreamSynchronize(stream_));
 }
@@ -621,6 +627,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   if (timing_) {
     LH_HIP(hipEventCreate(&ev.a));
     LH_HIP(hipEventCreate(&ev.b));
+    (cap_evs_ ? *cap_evs_ : evs_).push_back(ev);
   }
   const int newt = (int)far_ewt_[p].size(), newp = (int)far_ewp_[p].size();
   if (fp32) {
@@ -630,7 +637,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
     if (newp > 0)
       launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                     d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
-    if (timing_) LH_HIP(hipEventRecord(ev.a, st));
+    if (timing_) record(ev.a);
     if (sym) {
       launch_far_sym32(st, true, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
                        d_src32_.as<Src32>(), d_part_.as<double2>());
@@ -651,7 +658,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
-    if (timing_) LH_HIP(hipEventRecord(ev.a, st));
+    if (timing_) record(ev.a);
     if (rot) {
       launch_far_symrot64(st, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
                           d_src64_.as<Src64>(), d_part_.as<double2>());
@@ -675,10 +682,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
              d_ewl_.as<double>(), d_part_.as<double2>(), blk);
   fft_matvec(in_slot, fp32);
-  if (timing_) {
-    LH_HIP(hipEventRecord(ev.b, st));
-    evs_.push_back(ev);
-  }
+  if (timing_) record(ev.b);
   launch_reduce(st, nv, vi, sd, gs, which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
                 d_tab_.as<double>(), d_ainv_.as<double2>(), vec, in_slot, out_slot,
                 d_part_.as<double2>(), mode, d_norm_.as<double>());
@@ -732,20 +736,83 @@ void Engine::clear_events() {
   evs_.clear();
 }
 
-void Engine::far_times(double out[4]) {
-  LH_HIP(hipStreamSynchronize(stream_));
-  out[0] = out[1] = out[2] = out[3] = 0.0;
-  for (auto& e : evs_) {
+void Engine::set_timing(bool on) {
+  if (on != timing_) drop_graphs();   // captured graphs hold (or lack) event nodes
+  timing_ = on;
+}
+
+// An event on the engine's stream; during graph capture an event record
+// node of the graph (hipEventRecordExternal), re-recorded at every replay.
+void Engine::record(hipEvent_t e) {
+  if (cap_evs_) LH_HIP(hipEventRecordWithFlags(e, stream_, hipEventRecordExternal));
+  else LH_HIP(hipEventRecord(e, stream_));
+}
+
+void Engine::harvest(std::vector<EvPair>& evs, bool destroy) {
+  for (auto& e : evs) {
     float ms = 0.0f;
     LH_HIP(hipEventElapsedTime(&ms, e.a, e.b));
-    out[e.fp32 ? 1 : 0] += ms;
-    out[e.fp32 ? 3 : 2] += 1.0;
+    far_ms_[e.fp32 ? 1 : 0] += ms;
+    far_n_[e.fp32 ? 1 : 0] += 1;
+    if (destroy) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
   }
+  if (destroy) evs.clear();
+}
+
+void Engine::far_times(double out[4]) {
+  LH_HIP(hipStreamSynchronize(stream_));
+  harvest(evs_, true);
+  out[0] = far_ms_[0];
+  out[1] = far_ms_[1];
+  out[2] = (double)far_n_[0];
+  out[3] = (double)far_n_[1];
+}
+
+// `cycles` restart cycles, every kernel of them, captured once and replayed;
+// solves that converge (or leave the Arnoldi loop) inside the graph skip the
+// rest of it at the cost of one flag read per work item.
+Engine::CycleGraph& Engine::cycle_graph(bool mixed, int cycles) {
+  const int key = (cycles << 2) | (timing_ ? 2 : 0) | (mixed ? 1 : 0);
+  auto it = graphs_.find(key);
+  if (it != graphs_.end()) return it->second;
+  CycleGraph cg;
+  hipStream_t st = stream_;
+  hipGraph_t g = nullptr;
+  LH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  cap_evs_ = &cg.evs;
+  try {
+    for (int c = 0; c < cycles; ++c) enqueue_cycle(mixed, false);
+  } catch (...) {
+    cap_evs_ = nullptr;
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    for (auto& e : cg.evs) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    throw;
+  }
+  cap_evs_ = nullptr;
+  LH_HIP(hipStreamEndCapture(st, &g));
+  const hipError_t e = hipGraphInstantiate(&cg.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    for (auto& ev : cg.evs) {
+      (void)hipEventDestroy(ev.a);
+      (void)hipEventDestroy(ev.b);
+    }
+    LH_HIP(e);
+  }
+  return graphs_.emplace(key, std::move(cg)).first->second;
 }
 
 void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
-  (void)use_graph;
   clear_events();
+  far_ms_[0] = far_ms_[1] = 0.0;
+  far_n_[0] = far_n_[1] = 0;
   hipStream_t st = stream_;
   const int nv = (int)vit_.size(), S = (int)sd_.size();
   const VecItem* vi = d_vit_.as<VecItem>();
@@ -768,31 +835,40 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   launch_check(st, S, sdd, gs, d_norm_.as<double>(), 1, 0, d_counter_.as<int>());
   LH_HIP(hipGetLastError());
   int running = read_counter();
-  // every cycle performs at least one Arnoldi step of every running solve,
-  // so at most max_iter + 1 cycles can start
+  // Every cycle performs at least one Arnoldi step of every running solve,
+  // so at most max_iter + 1 cycles can start. With graphs, each launch
+  // replays K cycles and is followed by one counter read: K doubles from 1
+  // up to kAutoGraphCycles by default (a batch that converges early wastes
+  // at most that many empty cycles), or is fixed, or covers max_iter.
   long long cycles = 0;
-  const bool graph = use_graph && tiling_.graph && !timing_;
-  const int gi = mixed ? 1 : 0;
-  if (graph && !cycle_graph_[gi]) {
-    // one restart cycle, every kernel of it, captured once and replayed;
-    // solves that leave the Arnoldi loop early skip the remaining steps
-    hipGraph_t g = nullptr;
-    LH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    enqueue_cycle(mixed, false);
-    LH_HIP(hipStreamEndCapture(st, &g));
-    hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    LH_HIP(e);
-    cycle_graph_[gi] = ex;
-  }
-  while (running > 0 && cycles++ <= max_iter + 1) {
-    if (graph) LH_HIP(hipGraphLaunch(cycle_graph_[gi], st));
-    else enqueue_cycle(mixed, true);
-    LH_HIP(hipGetLastError());
-    running = read_counter();
+  int launches = 0;
+  const bool graph = use_graph && tiling_.graph;
+  const int whole = (int)std::min<long long>(kMaxGraphCycles, max_iter / LH_MMAX + 1);
+  int next = 1;
+  while (running > 0 && cycles <= max_iter + 1) {
+    if (graph) {
+      const int k = tiling_.graph_cycles > 0 ? std::min(tiling_.graph_cycles, kMaxGraphCycles)
+                    : tiling_.graph_cycles < 0 ? std::max(whole, 1)
+                                               : next;
+      CycleGraph& g = cycle_graph(mixed, k);
+      LH_HIP(hipGraphLaunch(g.exec, st));
+      LH_HIP(hipGetLastError());
+      cycles += k;
+      next = std::min(2 * next, kAutoGraphCycles);
+      running = read_counter();
+      if (timing_) harvest(g.evs, false);
+    } else {
+      enqueue_cycle(mixed, true);
+      LH_HIP(hipGetLastError());
+      ++cycles;
+      running = read_counter();
+      if (timing_) harvest(evs_, true);
+    }
+    ++launches;
   }
   LH_HIP(hipStreamSynchronize(st));
+  last_launches = launches;
+  last_cycles_issued = cycles;
 }
 
 // One restart cycle for every running solve (iterative.rs:69-158): v_0 =

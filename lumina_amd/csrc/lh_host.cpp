@@ -253,6 +253,25 @@ void natural_spline(const double* xs, const double* ys, int n, double* y2) {
 
 // lattice.rs:75-147 (cell_measure, eta, reciprocal_vectors), lattice.rs:168-233
 // (point order) and the r-independent factors of ewald.rs:119-273.
+double max_phase_revolutions(const Geometry& g, const Problem& p) {
+  if (g.n == 0) return 0.0;
+  double lo[3] = {g.x[0], g.y[0], g.z[0]}, hi[3] = {g.x[0], g.y[0], g.z[0]};
+  for (int i = 0; i < g.n; ++i) {
+    lo[0] = std::min(lo[0], g.x[i]);
+    hi[0] = std::max(hi[0], g.x[i]);
+    lo[1] = std::min(lo[1], g.y[i]);
+    hi[1] = std::max(hi[1], g.y[i]);
+    lo[2] = std::min(lo[2], g.z[i]);
+    hi[2] = std::max(hi[2], g.z[i]);
+  }
+  double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                          (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  if (p.has_sub)
+    diag = 2.0 * diag + 2.0 * std::fabs(p.z_interface) +
+           2.0 * std::max(std::fabs(lo[2]), std::fabs(hi[2]));
+  return p.k * diag / (2.0 * M_PI);
+}
+
 void ewald_records(const Problem& p, std::vector<double>& out, int& nreal, int& nrecip,
                    double& eta) {
   out.clear();

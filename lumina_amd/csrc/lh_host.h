@@ -98,6 +98,15 @@ struct Problem {
   double kb[3] = {0, 0, 0};
 };
 
+// The hardware v_sin_f32 / v_cos_f32 take the phase in revolutions and are
+// valid for |x| <= 256 revolutions. The largest far-field phase of a solve
+// is k times its geometry's bounding-box diagonal; with a substrate the
+// image of a source lies at most 2 diag + 2 |z_interface| + 2 max |z| away.
+// FP32 far fields use the hardware functions only while every solve's
+// phase stays below kMaxHwRevolutions.
+constexpr double kMaxHwRevolutions = 200.0;
+double max_phase_revolutions(const Geometry& g, const Problem& p);
+
 // Ewald records of a lattice problem at its k: the real-space images
 // (EwReal, m1 outer, m2 inner over [-n_real, n_real]) followed by the
 // spectral terms (EwRecip, n1 outer, n2 inner, |q| < 1e-10 left out), as

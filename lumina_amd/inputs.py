@@ -492,3 +492,25 @@ def batch_particles(count: int = 64, seed: int = 0, spacing: float = 2.0):
             depol = depol_factors(axes)
         out.append(Particle(shape, mat, r, axes, pos, depol))
     return out
+
+
+def batch_workload(n_particles: int = 64, n_wl: int = 16, n_env: float = 1.33,
+                   spacing: float = 2.0, seed: int = 0):
+    """The batched-solve workload: ``batch_particles(n_particles, seed)`` in
+    water (n_env), each at n_wl wavelengths 400 + t 400 / (n_wl - 1) nm, FCD
+    with cell_size = spacing, x-polarised light along +z. One problem dict
+    per (particle, wavelength), particle-major, with the keys Batch and
+    PairSumSession take (positions, polarisability, k, use_fcd, cell_size)
+    plus n, wl, material, shape and particle (its index)."""
+    parts = batch_particles(n_particles, seed=seed, spacing=spacing)
+    wls = [400.0 + t * 400.0 / (n_wl - 1) for t in range(n_wl)] if n_wl > 1 else [600.0]
+    probs = []
+    for ip, p in enumerate(parts):
+        for lam in wls:
+            probs.append(dict(positions=p.positions,
+                              polarisability=polarisability_block(p.material, lam, spacing, n_env,
+                                                                  depol=p.depol),
+                              k=wavenumber(lam, n_env), use_fcd=True, cell_size=spacing,
+                              n=len(p.positions), wl=lam, material=p.material, shape=p.shape,
+                              particle=ip))
+    return probs

@@ -47,6 +47,16 @@ def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
+def _sized(a, want: int, what: str) -> np.ndarray:
+    """A flat complex copy of ``a`` that must hold exactly ``want`` values:
+    the C ABI reads and writes fixed lengths (3N per vector, 9 or 9N per
+    polarisability), so a wrong length is refused before any C call."""
+    v = _c128(a).reshape(-1)
+    if v.size != want:
+        raise ValueError(f"{what} has {v.size} values, expected {want}")
+    return v
+
+
 @dataclass
 class IncidentField:
     """types.rs:37-74. ``polarisation`` may be complex (circular light)."""
@@ -228,6 +238,8 @@ class PairSumSession:
 
     def set_wavelength(self, k: float, polarisability, delta_eps: complex = 0j):
         pol = _c128(polarisability).reshape(-1)
+        if pol.size not in (9, 9 * self.n):
+            raise ValueError(f"polarisability has {pol.size} values, expected 9 or {9 * self.n}")
         uniform = 1 if pol.size == 9 else 0
         d = complex(delta_eps)
         check(self._lib.lh_pairsum_set_wavelength(self._h, float(k), cptr(pol), uniform,
@@ -245,7 +257,7 @@ class PairSumSession:
 
     def solve_gmres(self, rhs, tolerance: float, max_iterations: int):
         """Returns (x, info). Raises ConvergenceFailure (with .x and .info)."""
-        b = _c128(rhs).reshape(-1)
+        b = _sized(rhs, 3 * self.n, "rhs")
         x = np.empty_like(b)
         info = _lib.SolveInfo()
         st = self._lib.lh_pairsum_solve(self._h, cptr(b), float(tolerance), int(max_iterations),
@@ -265,7 +277,7 @@ class PairSumSession:
 
     def cross_sections(self, p, e_inc, amplitude: float = 1.0):
         out = np.zeros(3)
-        pv, ev = _c128(p).reshape(-1), _c128(e_inc).reshape(-1)
+        pv, ev = _sized(p, 3 * self.n, "p"), _sized(e_inc, 3 * self.n, "e_inc")
         check(self._lib.lh_pairsum_cross_sections(self._h, cptr(pv), cptr(ev), float(amplitude),
                                                   dptr(out)))
         return out

@@ -38,6 +38,7 @@ def test_struct_layout_matches_c(tmp_path):
         "lh_c64": _lib.c64, "lh_device_info": _lib.DeviceInfo,
         "lh_pairsum_spec": _lib.PairsumSpec, "lh_incident": _lib.Incident,
         "lh_solve_info": _lib.SolveInfo, "lh_batch_problem": _lib.BatchProblem,
+        "lh_kernel_times": _lib.KernelTimes,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lumina_hip.h"',
              "int main(void) {"]

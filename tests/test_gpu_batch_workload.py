@@ -1,0 +1,134 @@
+"""The batched-solve workload on the device: 64 Au/Ag spheres and
+ellipsoids (numpy PCG64 seed 0, N = 17 .. 13,949, anisotropic ellipsoid
+polarisabilities of types.rs:286-317) in water, each at 16 wavelengths
+400..800 nm, 1,024 GMRES(30) solves in one batch (the rayon sweep of
+runner.rs:145-218 as one device batch).
+
+* the whole batch in mixed precision and in FP64 at tol 1e-8, max_iter 1000:
+  physical cross sections (C_ext, C_abs > 0, C_sca = C_ext - C_abs >= 0
+  before the clamp of cda/mod.rs:229), the same convergence outcome in both
+  precisions, and mixed C_ext within 1e-6 of FP64 on every solve both
+  converge;
+* the largest particle's FP64 and FP32 matvec rows against the oracle's
+  on-the-fly matvec (assembly.rs:209-318);
+* sampled solves against the oracle's GMRES (iterative.rs:33-160): Au and
+  Ag, spheres and ellipsoids; converging ones at tol 1e-12, and the ones
+  GMRES(30) cannot converge within 1,000 iterations after exactly one restart
+  cycle (max_iter 30), whose iterate is the unique minimiser over the
+  30-dimensional Krylov space.
+"""
+import numpy as np
+import pytest
+
+from lumina_amd import inputs as I
+from lumina_amd.solver import FP64, MIXED, Batch, ConvergenceFailure
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def workload():
+    return I.batch_workload()
+
+
+def run_batch(backend, probs, tol, max_iter, prec):
+    b = Batch(backend, probs)
+    try:
+        b.run(tol, max_iter, prec)
+        try:
+            cs, infos, _ = b.results()
+        except ConvergenceFailure as e:
+            cs, infos = e.cross_sections, e.infos
+    finally:
+        b.close()
+    return cs, infos
+
+
+@pytest.mark.timeout(600)
+def test_whole_batch_mixed_against_fp64(backend, workload):
+    probs = workload
+    assert len(probs) == 1024
+    cs_m, inf_m = run_batch(backend, probs, 1e-8, 1000, MIXED)
+    cs_d, inf_d = run_batch(backend, probs, 1e-8, 1000, FP64)
+    conv_m = np.array([i["status"] == 0 for i in inf_m])
+    conv_d = np.array([i["status"] == 0 for i in inf_d])
+    it_m = np.array([i["iterations"] for i in inf_m])
+    it_d = np.array([i["iterations"] for i in inf_d])
+    for cs in (cs_m, cs_d):
+        assert np.all(np.isfinite(cs))
+        assert np.all(cs[:, 0] > 0) and np.all(cs[:, 1] > 0)
+    # energy balance on converged solves: C_ext - C_abs is the (clamped) C_sca
+    both = conv_m & conv_d
+    sca = cs_d[both, 0] - cs_d[both, 1]
+    assert np.all(sca > -1e-6 * cs_d[both, 0])
+    assert np.allclose(cs_d[both, 2], np.maximum(sca, 0.0))
+    # same outcome in both precisions; a solve may converge in one and not
+    # the other only within a few iterations of max_iter
+    differ = np.flatnonzero(conv_m != conv_d)
+    print(f"converged: mixed {conv_m.sum()}, fp64 {conv_d.sum()}, both {both.sum()}, "
+          f"differ {len(differ)} (iterations {list(zip(it_m[differ], it_d[differ]))})")
+    assert len(differ) <= 8
+    assert np.all(np.minimum(it_m[differ], it_d[differ]) >= 900)
+    err = np.abs(cs_m[both, 0] - cs_d[both, 0]) / cs_d[both, 0]
+    print(f"mixed vs fp64 C_ext: max {err.max():.2e}, median {np.median(err):.2e} over "
+          f"{both.sum()} solves")
+    assert err.max() < 1e-6
+    # every class of the workload is among the converged solves
+    classes = {(probs[q]["material"], probs[q]["shape"]) for q in np.flatnonzero(both)}
+    assert classes == {("Au", "sphere"), ("Au", "ellipsoid"), ("Ag", "sphere"),
+                       ("Ag", "ellipsoid")}
+    # iteration counts of the two precisions stay close on converged solves
+    assert np.median(np.abs(it_m[both] - it_d[both])) <= 2
+
+
+def test_largest_particle_matvec_rows(backend, workload):
+    # particle 15: the Ag sphere of 13,949 dipoles, at 613 nm
+    p = workload[16 * 15 + 8]
+    n = p["n"]
+    assert n == max(q["n"] for q in workload) == 13949
+    s = backend.create_pairsum_session(p["positions"], p["polarisability"], p["k"], True, 2.0)
+    r = np.random.default_rng(21)
+    x = r.standard_normal(3 * n) + 1j * r.standard_normal(3 * n)
+    y64 = s.matvec(x, FP64)
+    y32 = s.matvec(x, MIXED)
+    s.close()
+    P = O.Problem(p["positions"], p["polarisability"], p["k"], True, 2.0)
+    for lo in np.linspace(0, n - 32, 6).astype(int):
+        yo = P.matvec_rows(x, int(lo), int(lo) + 32)
+        sl = slice(3 * lo, 3 * lo + 96)
+        e64 = np.max(np.abs(y64[sl] - yo)) / np.max(np.abs(yo))
+        e32 = np.max(np.abs(y32[sl] - yo)) / np.max(np.abs(yo))
+        print(f"N={n} rows {lo}..{lo + 31}: fp64 {e64:.1e}, fp32 far field {e32:.1e}")
+        assert e64 < 1e-12
+        assert e32 < 2e-6
+
+
+# (particle index, wavelength index): converging at tol 1e-12
+CONVERGING = [(12, 0), (12, 5), (16, 0), (16, 5), (25, 0)]
+# (particle index, wavelength index): one restart cycle
+ONE_CYCLE = [(14, 5), (55, 10), (63, 15), (46, 10)]
+
+
+def test_sampled_solves_against_oracle(backend, workload):
+    probs = [workload[16 * a + t] for a, t in CONVERGING]
+    cs, infos = run_batch(backend, probs, 1e-12, 1000, FP64)
+    for p, c, info in zip(probs, cs, infos):
+        P = O.Problem(p["positions"], p["polarisability"], p["k"], True, 2.0)
+        co, _, io = P.compute_cross_sections(1e-12, 1000, nthreads=16)
+        assert io["status"] == 0
+        e = np.max(np.abs(c - co) / np.abs(co[0]))
+        print(f"{p['material']} {p['shape']} N={p['n']} {p['wl']:.1f} nm: iterations "
+              f"{info['iterations']} (oracle {io['iterations']}), rel {e:.1e}")
+        assert info["status"] == 0
+        assert e < 1e-10
+    probs = [workload[16 * a + t] for a, t in ONE_CYCLE]
+    cs, infos = run_batch(backend, probs, 1e-12, 30, FP64)
+    for p, c, info in zip(probs, cs, infos):
+        P = O.Problem(p["positions"], p["polarisability"], p["k"], True, 2.0)
+        co, _, io = P.compute_cross_sections(1e-12, 30, nthreads=16)
+        e = np.max(np.abs(c[:2] - co[:2]) / np.abs(co[0]))
+        print(f"{p['material']} {p['shape']} N={p['n']} {p['wl']:.1f} nm, one cycle: "
+              f"residual {info['residual']:.2e}, rel {e:.1e}")
+        assert info["status"] == 5 and info["iterations"] == 30
+        assert e < 1e-9

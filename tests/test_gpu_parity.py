@@ -129,6 +129,34 @@ def test_matvec_fp32_far_field(backend, sincos, monkeypatch):
     assert e < 2e-6
 
 
+@pytest.mark.parametrize("form", ["256", "512"])
+def test_set_wavelength_beyond_hardware_sincos_range(backend, form, monkeypatch):
+    """A session built at a k whose far-field phases lie within the hardware
+    sin/cos range (v_sin_f32: 256 revolutions) and re-targeted to a k beyond
+    it switches the FP32 far field to the software sincos_f32 (and off the
+    unordered-pair form, block 512, which uses the hardware functions)."""
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", form)
+    r = np.random.default_rng(11)
+    n = 200
+    pos = np.stack([np.sort(r.uniform(0.0, 6000.0, n)), r.uniform(-5, 5, n),
+                    r.uniform(-5, 5, n)], axis=1)
+    alpha = I.isotropic_block(40.0 + 8.0j)
+    k1, k2 = 2 * np.pi / 1000.0, 2 * np.pi / 20.0   # 6 and 300 revolutions over 6 um
+    s = backend.create_pairsum_session(pos, alpha, k1, use_fcd=False, cell_size=2.0,
+                                       precision=MIXED)
+    x = rand_x(n, 12)
+    e1 = rel(s.matvec(x, MIXED), O.Problem(pos, alpha, k1, False, 2.0).matvec(x))
+    s.set_wavelength(k2, alpha)
+    yo = O.Problem(pos, alpha, k2, False, 2.0).matvec(x)
+    e2 = rel(s.matvec(x, MIXED), yo)
+    e64 = rel(s.matvec(x, FP64), yo)
+    print(f"form {form}: fp32 rel err at 6 rev {e1:.2e}, at 300 rev {e2:.2e}; fp64 {e64:.2e}")
+    assert e1 < 2e-5
+    # FP32 phases of up to 1,900 rad carry ~1e-4 relative rounding
+    assert e2 < 2e-3
+    assert e64 < 1e-11
+
+
 def test_solve_fp64_parity_sphere(backend):
     pos, alpha, k, cell = sphere_problem()
     s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell)
@@ -253,13 +281,19 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
             ("b64", {"LUMINA_FAR_BLOCK": "64"}), ("b128", {"LUMINA_FAR_BLOCK": "128"}),
             ("b256", {"LUMINA_FAR_BLOCK": "256"}), ("sym", {"LUMINA_FAR_BLOCK": "512"}),
             ("rot", {"LUMINA_FAR_BLOCK": "1024"}),
-            ("cgs2", {"LUMINA_CGS2": "1", "LUMINA_FAR_BLOCK": "256"})]
+            ("cgs2", {"LUMINA_CGS2": "1", "LUMINA_FAR_BLOCK": "256"}),
+            ("k3", {"LUMINA_GRAPH_CYCLES": "3", "LUMINA_FAR_BLOCK": "256"}),
+            ("kall", {"LUMINA_GRAPH_CYCLES": "all", "LUMINA_FAR_BLOCK": "256"}),
+            ("timed", {"LUMINA_FAR_BLOCK": "256"})]
     for name, env in opts:
-        for key in ("LUMINA_GRAPH", "LUMINA_FAR_BLOCK", "LUMINA_CGS2"):
+        for key in ("LUMINA_GRAPH", "LUMINA_FAR_BLOCK", "LUMINA_CGS2", "LUMINA_GRAPH_CYCLES"):
             monkeypatch.delenv(key, raising=False)
         for key, val in env.items():
             monkeypatch.setenv(key, val)
         b = Batch(backend, probs)
+        if name == "timed":
+            # far-field events recorded as nodes of the captured graphs
+            b.set_timing(True)
         b.run(1e-10, 200, precision)
         try:
             cs, infos, p = b.results(want_moments=True)
@@ -282,6 +316,22 @@ def test_batch_launch_options_bitwise(backend, precision, monkeypatch):
                 assert kt["far32_block"] == 256
         elif name == "cgs2":
             assert kt["far64_block"] == 256
+        elif name in ("k3", "kall", "timed"):
+            assert kt["far64_block"] == 256
+            cyc = max(i["cycles"] for i in infos)
+            if name == "kall":
+                # 200 iterations: 7 cycles per launch
+                assert kt["host_round_trips"] == -(-cyc // 7)
+                assert kt["cycles_issued"] == 7 * kt["host_round_trips"]
+            elif name == "k3":
+                assert kt["host_round_trips"] == -(-cyc // 3)
+            else:
+                # 1, 2, 4, 8, 8, ... cycles per launch
+                assert kt["cycles_issued"] >= cyc
+                assert kt["host_round_trips"] <= 3 + -(-max(cyc - 7, 0) // 8)
+                n_fp = kt["far32_launches"] if precision == MIXED else kt["far64_launches"]
+                assert n_fp > 0 and kt["far64_launches"] > 0
+                assert (kt["far32_ms"] if precision == MIXED else kt["far64_ms"]) > 0
         else:
             assert kt["far64_block"] in (64, 128, 256, 512, 1024)
         runs[name] = (cs, [i["iterations"] for i in infos], p,

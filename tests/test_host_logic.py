@@ -75,6 +75,17 @@ int main() {
     for (int i = 0; i < n; ++i) std::scanf("%lf %lf", &xs[i], &ys[i]);
     natural_spline(xs.data(), ys.data(), n, y2.data());
     for (int i = 0; i < n; ++i) std::printf("%.17g\n", y2[i]);
+  } else if (mode == 6) {
+    // far-field phase range: n k has_sub z_interface, then positions
+    Geometry g;
+    Problem p;
+    int n;
+    std::scanf("%d %lf %d %lf", &n, &p.k, &p.has_sub, &p.z_interface);
+    g.n = n;
+    g.x.resize(n); g.y.resize(n); g.z.resize(n);
+    for (int i = 0; i < n; ++i) std::scanf("%lf %lf %lf", &g.x[i], &g.y[i], &g.z[i]);
+    const double rev = max_phase_revolutions(g, p);
+    std::printf("%.17g %d\n", rev, rev > kMaxHwRevolutions ? 1 : 0);
   } else {
     // Ewald records: a1 a2 eta nr nq kb k
     Problem p;
@@ -337,3 +348,32 @@ def test_natural_spline_matches_restatement(harness, name):
         assert np.array_equal(y2, np.array(spl.y2))
     y2 = [float(v) for v in run(harness, "4 2\n300 2.25\n1200 2.5\n")[:2]]
     assert y2 == [0.0, 0.0]
+
+
+def phase_range(exe, pos, k, sub=False, z0=0.0):
+    lines = [f"6 {len(pos)} {k!r} {1 if sub else 0} {z0!r}"]
+    lines += [" ".join(repr(float(v)) for v in p) for p in pos]
+    out = run(exe, "\n".join(lines) + "\n")
+    rev, off = out[0].split()
+    return float(rev), int(off) == 1
+
+
+def test_hardware_sincos_range_decision(harness):
+    """The FP32 far field leaves the hardware sin/cos (valid to 256
+    revolutions) once k times the bounding-box diagonal exceeds 200
+    revolutions; the engine takes this decision at build and again whenever
+    a solve is re-targeted to a new k (set_problem)."""
+    pos = np.array([[0.0, 0.0, 0.0], [3000.0, 4000.0, 0.0], [1000.0, 0.0, 0.0]])
+    diag = 5000.0
+    for lam in (500.0, 26.0, 24.0):
+        k = 2 * np.pi / lam
+        rev, off = phase_range(harness, pos, k)
+        assert abs(rev - diag / lam) < 1e-9 * rev
+        assert off == (diag / lam > 200.0)
+    # with a substrate the image of a source lies up to 2 diag + 2|z0| + 2 max|z| away
+    pos2 = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 5.0]])
+    k = 2 * np.pi / 10.0
+    rev, _ = phase_range(harness, pos2, k, sub=True, z0=-2.0)
+    assert abs(rev - (2 * 4.0 + 4.0 + 10.0) / 10.0) < 1e-12
+    # a single dipole has no phase range
+    assert phase_range(harness, pos[:1], k)[0] == 0.0
