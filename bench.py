@@ -57,17 +57,7 @@ def log(*a):
 
 def batch_problems(n_particles=64, n_wl=16, n_env=1.33, spacing=2.0):
     from lumina_amd import inputs as I
-    parts = I.batch_particles(n_particles, seed=0, spacing=spacing)
-    wls = [400.0 + t * 400.0 / (n_wl - 1) for t in range(n_wl)] if n_wl > 1 else [600.0]
-    probs = []
-    for p in parts:
-        for lam in wls:
-            probs.append(dict(positions=p.positions,
-                              polarisability=I.polarisability_block(p.material, lam, spacing,
-                                                                    n_env, depol=p.depol),
-                              k=I.wavenumber(lam, n_env), use_fcd=True, cell_size=spacing,
-                              n=len(p.positions), wl=lam))
-    return probs
+    return I.batch_workload(n_particles, n_wl, n_env, spacing)
 
 
 def pairs(n):
@@ -75,59 +65,113 @@ def pairs(n):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it: model name, family, model,
+    stepping."""
+    keys = {"model name": None, "cpu family": None, "model": None, "stepping": None}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in keys and keys[k] is None:
+                    keys[k] = v.strip()
+                if line.strip() == "":
+                    break
     except OSError:
         pass
-    return "unknown"
+    name = keys["model name"] or "unknown"
+    return (f"{name} (family {keys['cpu family']}, model {keys['model']}, "
+            f"stepping {keys['stepping']})")
 
 
-def cpu_baseline(probs, budget_s=20.0):
-    """The oracle (C++ FP64 restatement of the reference CPU path: rows of
-    the on-the-fly matvec over a thread pool, serial GMRES) on a bounded
-    sample: full solves of the batch's problems at the middle wavelength with
-    N <= 2000, smallest first, until the time budget is used (the value), and
-    the matvec rate on the median-size problem with all threads and with one
-    (median of 3 after a warm-up)."""
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU this process may run on (its
+    affinity mask), at most 16 -- one GPU's share of the host's cores."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def host_cpus():
+    out = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads_used": cpu_threads()}
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        out["affinity"] = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            out["cgroup_cpu_max"] = f.read().strip()
+    except OSError:
+        out["cgroup_cpu_max"] = None
+    return out
+
+
+def time_rows(P, x, n, rows, nthreads):
+    """Seconds for `rows` target rows of the oracle's on-the-fly matvec, as
+    nthreads blocks spread evenly over the dipoles, one block per thread
+    (the ctypes call releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    per = max(1, -(-rows // nthreads))
+    nb = max(1, min(nthreads, n // per)) if per <= n else 1
+    per = min(per, n)
+    starts = np.linspace(0, n - per, nb).astype(int)
+    with ThreadPoolExecutor(nb) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda a: P.matvec_rows(x, int(a), int(a) + per), starts))
+        dt = time.perf_counter() - t0
+    return dt, nb * per
+
+
+def cpu_baseline(probs, infos64, budget_s=24.0):
+    """The oracle (C++ FP64 restatement of the reference CPU path:
+    matvec_on_the_fly's rows over a thread pool, assembly.rs:224-226) on the
+    batch's own size mix. For every particle (at its middle wavelength) a
+    sample of target rows spread over its dipoles is timed with all threads
+    and with one, giving its row rate; the CPU time of the whole workload is
+    then sum over solves of matvecs x N(N-1) / rate, with each solve's FP64
+    matvec count (iterations + 2 cycles, as the reference's GMRES performs
+    them). value = the workload's pair evaluations / that time. Extrapolated
+    from the sampled rows; GMRES vector work (serial in the reference) is not
+    timed."""
     from oracle import pyoracle as O
-    threads = min(16, os.cpu_count() or 1)
-    wls = sorted(set(p["wl"] for p in probs))
-    mid = wls[len(wls) // 2]
-    at_mid = sorted([p for p in probs if p["wl"] == mid], key=lambda p: p["n"])
-    sample = [p for p in at_mid if p["n"] <= 2000]
-    t0 = time.perf_counter()
-    ev, nsolve = 0.0, 0
-    for p in sample:
+    threads = cpu_threads()
+    parts = {}
+    for q, p in enumerate(probs):
+        parts.setdefault(p["particle"], []).append(q)
+    per_part = budget_s / (1.5 * len(parts))
+    rate_all, rate_one = {}, {}
+    t_start = time.perf_counter()
+    rng = np.random.default_rng(0)
+    for ip, qs in parts.items():
+        p = probs[qs[len(qs) // 2]]
+        n = p["n"]
         P = O.Problem(p["positions"], p["polarisability"], p["k"], True, p["cell_size"])
-        _, _, info = P.compute_cross_sections(1e-8, 1000, nthreads=threads)
-        ev += info["matvecs"] * pairs(p["n"])
-        nsolve += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    med = at_mid[len(at_mid) // 2]
-    P = O.Problem(med["positions"], med["polarisability"], med["k"], True, med["cell_size"])
-    x = np.random.default_rng(0).standard_normal(3 * med["n"]) + 0j
-    rates = {}
-    for nt in (threads, 1):
-        P.matvec(x, nthreads=nt)
-        ts = []
-        for _ in range(3):
-            t1 = time.perf_counter()
-            P.matvec(x, nthreads=nt)
-            ts.append(time.perf_counter() - t1)
-        rates[nt] = pairs(med["n"]) / float(np.median(ts))
-    return {"value": ev / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(), "solves_per_s": nsolve / dt,
-            "matvec_pair_evals_per_s": {"all_threads": rates[threads], "one_thread": rates[1],
-                                        "n": med["n"], "threads": threads},
-            "sample": f"{nsolve} full GMRES solves (tol 1e-8, FP64, FCD) of the batch's "
-                      f"problems at {mid:.1f} nm with N <= 2000, smallest first; "
-                      f"{ev:.3g} pair-evals in {dt:.1f} s; matvec rates: median of 3 on the "
-                      f"N={med['n']} problem"}
+        x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
+        for nt, budget, store in ((threads, per_part, rate_all), (1, 0.5 * per_part, rate_one)):
+            rows = min(n, 4 * nt)
+            dt, done = time_rows(P, x, n, rows, nt)
+            rows = int(min(n, max(rows, rows * budget / max(dt, 1e-6))))
+            dt, done = time_rows(P, x, n, rows, nt)
+            store[ip] = done * (n - 1) / dt
+    work = t_all = t_one = 0.0
+    for q, p in enumerate(probs):
+        i = infos64[q]
+        w = (i["iterations"] + 2 * i["cycles"]) * pairs(p["n"])
+        work += w
+        t_all += w / rate_all[p["particle"]]
+        t_one += w / rate_one[p["particle"]]
+    hc = host_cpus()
+    return {"value": work / t_all, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+            "extrapolated": True, **hc,
+            "one_thread_value": work / t_one, "workload_cpu_s_all_threads": t_all,
+            "workload_cpu_s_one_thread": t_one, "workload_pair_evals": work,
+            "sample": f"oracle matvec rows of each of the {len(parts)} particles (middle "
+                      f"wavelength) spread over its dipoles, on {threads} threads and on 1; "
+                      f"the whole {len(probs)}-solve workload's FP64 matvecs (iterations + 2 "
+                      f"cycles each) priced at those row rates (extrapolated); "
+                      f"{time.perf_counter() - t_start:.1f} s of sampling"}
 
 
 def cpu_matvec_rate(P, n, pairs_per_matvec, rows=None, reps=3):
@@ -136,7 +180,7 @@ def cpu_matvec_rate(P, n, pairs_per_matvec, rows=None, reps=3):
     ``rows`` target rows spread over a thread pool (sampled, scaled by
     rows / n)."""
     from concurrent.futures import ThreadPoolExecutor
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     x = np.random.default_rng(0).standard_normal(3 * n) + 0j
     if rows is None:
         P.matvec(x, nthreads=threads)
@@ -147,7 +191,7 @@ def cpu_matvec_rate(P, n, pairs_per_matvec, rows=None, reps=3):
             ts.append(time.perf_counter() - t0)
         dt = float(np.median(ts))
         return {"value": pairs_per_matvec / dt, "unit": "pair-evals/s", "cores": threads,
-                "kind": "port", "cpu_model": cpu_model(),
+                "kind": "port", **host_cpus(),
                 "sample": f"median of {reps} full oracle matvecs (N={n}) after a warm-up, "
                           f"{dt:.2f} s each"}
     # `threads` blocks of rows spread evenly over the dipoles
@@ -162,7 +206,7 @@ def cpu_matvec_rate(P, n, pairs_per_matvec, rows=None, reps=3):
         list(ex.map(work, chunks))
         dt = time.perf_counter() - t0
     return {"value": pairs_per_matvec * rows / n / dt, "unit": "pair-evals/s", "cores": threads,
-            "kind": "port", "cpu_model": cpu_model(),
+            "kind": "port", "extrapolated": True, **host_cpus(),
             "sample": f"oracle matvec, {threads} blocks of {per} rows spread over N={n}, on "
                       f"{threads} threads in {dt:.2f} s, scaled to the full matvec (extrapolated)"}
 
@@ -178,114 +222,166 @@ def results_allow_failures(batch):
     return cs, infos
 
 
+def far_kernel_name(prec_mixed, blk):
+    # block 512: the one-tensor-per-unordered-pair kernels with LDS-staged
+    # sources (with and without the exclusion test); 1024: the FP64 one whose
+    # sources rotate around the wave; otherwise the row kernels
+    hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
+    if blk == 512:
+        return "k_far_sym32<true|false>" if prec_mixed else "k_far_sym64<true|false>"
+    if blk == 1024:
+        return "k_far_sym<double>"
+    return (f"k_far_row<float, false, {str(hw).lower()}, true|false>" if prec_mixed
+            else "k_far_row<double, false, false, true|false>")
+
+
 def run_batch(args):
     from lumina_amd.solver import (FP64, MIXED, Batch, ConvergenceFailure, HipBackend,
                                    IncidentField, PairSumSession)
     probs = batch_problems(args.particles, args.wavelengths)
     S = len(probs)
+    nw = args.wavelengths
     log(f"[bench] batch: {S} solves, N in [{min(p['n'] for p in probs)}, "
         f"{max(p['n'] for p in probs)}], total dipoles {sum(p['n'] for p in probs)}")
     be = HipBackend(0)
     t0 = time.perf_counter()
     batch = Batch(be, probs)
     log(f"[bench] batch resident in HBM ({time.perf_counter() - t0:.1f} s to build)")
+    # far-field HIP events, recorded as nodes of the captured cycle graphs:
+    # the timed path is the graph path that ships
     batch.set_timing(True)
 
     prec = MIXED if args.precision == "mixed" else FP64
     for w in range(args.warmup):
         batch.run(args.tol, args.max_iter, prec)
         log(f"[bench] warmup {w} done")
-    times, evals, solves = [], 0.0, 0
+    times, evals = [], 0.0
     kt_tot = {"far32_ms": 0.0, "far32_pairs": 0.0, "far32_launches": 0,
               "far64_ms": 0.0, "far64_pairs": 0.0, "far64_launches": 0}
-    for s in range(args.steps):
+    trips = []
+    for s_ in range(args.steps):
         t1 = time.perf_counter()
         batch.run(args.tol, args.max_iter, prec)
         dt = time.perf_counter() - t1
         times.append(dt)
         cs_mixed, infos = results_allow_failures(batch)
         evals += sum(i["pair_evals"] for i in infos)
-        solves += S
         kt = batch.kernel_times()
         for key in kt_tot:
             kt_tot[key] += kt[key]
+        trips.append((kt["host_round_trips"], kt["cycles_issued"]))
         blocks = (kt["far64_block"], kt["far32_block"], kt["tune_ms"])
-        log(f"[bench] step {s}: {dt:.2f} s, iters mean "
-            f"{np.mean([i['iterations'] for i in infos]):.1f}, max {max(i['iterations'] for i in infos)}")
+        log(f"[bench] step {s_}: {dt:.2f} s, iters mean "
+            f"{np.mean([i['iterations'] for i in infos]):.1f}, max "
+            f"{max(i['iterations'] for i in infos)}, {kt['host_round_trips']} graph launches "
+            f"({kt['cycles_issued']} cycles)")
     total = sum(times)
     value = evals / total
     iters = [i["iterations"] for i in infos]
-
     conv = np.array([i["status"] == 0 for i in infos])
     batch.close()
-    # FP64 answer: the problems at 4 of the wavelengths, solved as one FP64
-    # batch at the same tolerance
-    err, fp64_batch = None, None
+
+    # the whole solve as one graph launch (every cycle up to max_iter in one
+    # captured graph), one warm-up and one timed run
+    one_launch = None
+    if not args.no_one_launch:
+        os.environ["LUMINA_GRAPH_CYCLES"] = "all"
+        if blocks[0] == 512 and blocks[1] == 512:
+            os.environ["LUMINA_FAR_BLOCK"] = "sym"   # the forms the timed batch chose
+        b1 = Batch(be, probs)
+        os.environ.pop("LUMINA_GRAPH_CYCLES")
+        os.environ.pop("LUMINA_FAR_BLOCK", None)
+        b1.run(args.tol, args.max_iter, prec)
+        t1 = time.perf_counter()
+        b1.run(args.tol, args.max_iter, prec)
+        dt1 = time.perf_counter() - t1
+        cs1, inf1 = results_allow_failures(b1)
+        kt1 = b1.kernel_times()
+        b1.close()
+        one_launch = {"pair_evals_per_s": sum(i["pair_evals"] for i in inf1) / dt1,
+                      "s": dt1, "host_round_trips": kt1["host_round_trips"],
+                      "cycles_issued": kt1["cycles_issued"],
+                      "identical_to_timed_path": bool(np.array_equal(cs1, cs_mixed))}
+        log(f"[bench] one graph launch per solve: {dt1:.2f} s, "
+            f"{kt1['host_round_trips']} launch(es), same results: "
+            f"{one_launch['identical_to_timed_path']}")
+
+    # FP64 answer of every solve, one FP64 batch at the same tolerance (the
+    # FP64 far-field form the mixed batch's tuner chose for its residuals)
+    err, fp64_batch, infos64 = None, None, None
     if not args.no_error:
-        nw = args.wavelengths
-        widx = sorted(set([0, nw // 3, (2 * nw) // 3, nw - 1]))
-        sel = [q for q in range(S) if (q % nw) in widx]
-        b64 = Batch(be, [probs[q] for q in sel])
+        os.environ["LUMINA_FAR_BLOCK"] = {512: "sym"}.get(blocks[0], str(blocks[0]))
+        b64 = Batch(be, probs)
+        os.environ.pop("LUMINA_FAR_BLOCK")
         t1 = time.perf_counter()
         b64.run(args.tol, args.max_iter, FP64)
         dt64 = time.perf_counter() - t1
         cs_ref, infos64 = results_allow_failures(b64)
         b64.close()
-        ok = conv[sel] & np.array([i["status"] == 0 for i in infos64])
-        errs = np.abs(cs_mixed[sel, 0] - cs_ref[:, 0]) / np.abs(cs_ref[:, 0])
+        conv64 = np.array([i["status"] == 0 for i in infos64])
+        ok = conv & conv64
+        errs = np.abs(cs_mixed[:, 0] - cs_ref[:, 0]) / np.abs(cs_ref[:, 0])
         err = float(np.max(errs[ok])) if ok.any() else None
-        fp64_batch = {"pair_evals_per_s": sum(i["pair_evals"] for i in infos64) / dt64,
-                      "solves_per_s": len(sel) / dt64, "s": dt64, "solves": len(sel),
-                      "converged": int(sum(i["status"] == 0 for i in infos64)),
-                      "sample": f"the problems at wavelength indices {widx} of {nw}"}
-        log(f"[bench] batched FP64 ({len(sel)} solves): {dt64:.1f} s; max C_ext err "
+        fail = ~conv & ~conv64
+        one = conv != conv64
+        fp64_batch = {
+            "pair_evals_per_s": sum(i["pair_evals"] for i in infos64) / dt64,
+            "solves_per_s": S / dt64, "s": dt64, "solves": S, "converged": int(conv64.sum()),
+            "note": "a single run: its time includes the graph capture",
+            "cext_err_solves": int(ok.sum()),
+            "cext_rel_err_median_vs_fp64": float(np.median(errs[ok])) if ok.any() else None,
+            "max_iter_rows": {
+                "both_precisions": int(fail.sum()),
+                "cext_rel_spread_max": float(np.max(errs[fail])) if fail.any() else None,
+                "cext_rel_spread_median": float(np.median(errs[fail])) if fail.any() else None},
+            "converged_in_one_precision_only": [
+                {"solve": int(q), "particle": probs[q]["particle"], "n": probs[q]["n"],
+                 "wl": probs[q]["wl"], "mixed": [infos[q]["iterations"], infos[q]["residual"]],
+                 "fp64": [infos64[q]["iterations"], infos64[q]["residual"]]}
+                for q in np.flatnonzero(one)]}
+        log(f"[bench] batched FP64 ({S} solves): {dt64:.1f} s; max C_ext err "
             f"(both converged, {int(ok.sum())} solves): {err}")
 
-    # plain FP64: one solve per launch sequence, a subset of the same problems
+    # plain FP64: one session and solve per problem, the problems at
+    # args.plain_wavelengths of the wavelengths (every particle)
     plain = None
     if not args.no_fp64_plain:
-        sub = probs[args.wavelengths // 2::args.wavelengths][: args.plain_count]
-        t_pl, ev_pl = 0.0, 0.0
+        npw = max(1, min(args.plain_wavelengths, nw))
+        widx = sorted(set(np.linspace(0, nw - 1, npw).round().astype(int).tolist()))
+        sub = [p for q, p in enumerate(probs) if (q % nw) in widx]
+        t_pl, ev_pl, conv_pl = 0.0, 0.0, 0
         # the far-field form the batch's tuning chose for FP64, so the sessions
         # below do not each time the candidates inside the solve ("sym" selects
         # the unordered-pair form for FP64 as well: "512" alone keeps the FP64
         # row kernel)
         fp64_form = {512: "sym"}.get(blocks[0], str(blocks[0]))
-        os.environ.setdefault("LUMINA_FAR_BLOCK", fp64_form)
+        os.environ["LUMINA_FAR_BLOCK"] = fp64_form
         for p in sub:
             s_ = PairSumSession(be, p["positions"], p["polarisability"], p["k"], True,
                                 p["cell_size"], precision=FP64)
             t1 = time.perf_counter()
             try:
                 out, _, info = s_.compute_cross_sections(IncidentField(), args.tol, args.max_iter)
+                conv_pl += 1
             except ConvergenceFailure as e:
                 info = e.info
             t_pl += time.perf_counter() - t1
             ev_pl += info["pair_evals"]
             s_.close()
+        os.environ.pop("LUMINA_FAR_BLOCK")
         plain = {"pair_evals_per_s": ev_pl / t_pl, "solves_per_s": len(sub) / t_pl,
-                 "solves": len(sub), "far64_form": os.environ.get("LUMINA_FAR_BLOCK"),
-                 "sample": f"{len(sub)} problems (each particle at the middle wavelength), "
-                           f"one FP64 session and solve each, tol {args.tol:g}"}
-        log(f"[bench] plain FP64: {plain['pair_evals_per_s']:.3e} pair-evals/s")
+                 "solves": len(sub), "converged": conv_pl, "far64_form": fp64_form,
+                 "sample": f"{len(sub)} problems (every particle at wavelength indices {widx} "
+                           f"of {nw}), one FP64 session and solve each, tol {args.tol:g}"}
+        log(f"[bench] plain FP64: {plain['pair_evals_per_s']:.3e} pair-evals/s on {len(sub)} solves")
 
     far_ms = kt_tot["far32_ms"] if prec == MIXED else kt_tot["far64_ms"]
     far_pairs = kt_tot["far32_pairs"] if prec == MIXED else kt_tot["far64_pairs"]
     far_n = kt_tot["far32_launches"] if prec == MIXED else kt_tot["far64_launches"]
     achieved = FLOPS_PER_FAR_PAIR * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else 0.0
     peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
-    hw = os.environ.get("LUMINA_FP32_SINCOS", "hw") != "sw"
-    # block 512: the one-tensor-per-unordered-pair kernels with LDS-staged
-    # sources (with and without the exclusion test); 1024: the FP64 one whose
-    # sources rotate around the wave; otherwise the row kernels
     blk = blocks[1] if prec == MIXED else blocks[0]
-    if blk == 512:
-        kname = "k_far_sym32<true|false>" if prec == MIXED else "k_far_sym64<true|false>"
-    elif blk == 1024:
-        kname = "k_far_sym<double>"
-    else:
-        kname = (f"k_far_row<float, false, {str(hw).lower()}, true|false>" if prec == MIXED
-                 else "k_far_row<double, false, false, true|false>")
+    kname = far_kernel_name(prec == MIXED, blk)
     # HBM bytes per launch of this kernel on this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy)
     traffic = None
@@ -297,9 +393,10 @@ def run_batch(args):
     roof = {"bound": "valu", "kernel": kname,
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
-            "flops_per_pair": FLOPS_PER_FAR_PAIR,
-            "targets_per_item": blocks[1] if prec == MIXED else blocks[0],
-            "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0}
+            "flops_per_pair": FLOPS_PER_FAR_PAIR, "targets_per_item": blk,
+            "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0,
+            "events": "HIP events on the batch stream, event-record nodes of the captured "
+                      "cycle graphs, around every far-field launch of the timed steps"}
 
     line = {
         "metric": "Green pair-evals/s (batched mixed-precision CDA solves)",
@@ -308,23 +405,31 @@ def run_batch(args):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 far field + fp64 Krylov" if prec == MIXED else "fp64",
         "data": "synthetic (seeded discretised particles, J&C Au/Ag)",
-        "config": {"workload": f"{args.particles} Au/Ag spheres+ellipsoids x {args.wavelengths} "
+        "config": {"workload": f"{args.particles} Au/Ag spheres+ellipsoids x {nw} "
                                f"wavelengths 400-800 nm, d=2 nm, FCD, water, GMRES(30) "
                                f"tol {args.tol:g}, one batched solve per step",
                    "solves": S, "dipoles": int(sum(p["n"] for p in probs)),
                    "n_min": int(min(p["n"] for p in probs)),
                    "n_max": int(max(p["n"] for p in probs)), "parallelism": "single GPU"},
-        "solves_per_s": solves / total,
+        "solves_per_s": S * args.steps / total,
+        "converged_solves_per_s": int(conv.sum()) * args.steps / total,
+        "converged": int(conv.sum()), "failed_max_iter": int((~conv).sum()),
         "iterations_mean": float(np.mean(iters)), "iterations_max": int(max(iters)),
-        "converged": int(conv.sum()), "max_iter": args.max_iter,
-        "cext_rel_err_max_vs_fp64": err, "fp64_batched": fp64_batch,
-        "fp64_plain": plain,
+        "max_iter": args.max_iter,
+        "host_round_trips_per_step": trips[-1][0] if trips else None,
+        "cycles_issued_per_step": trips[-1][1] if trips else None,
+        "graph_cycles_per_launch": "1, 2, 4, then 8 (LUMINA_GRAPH_CYCLES unset)",
+        "cext_rel_err_max_vs_fp64": err,
+        "cext_err_covers": (f"{fp64_batch['cext_err_solves']} of {S} solves (those that "
+                            f"converge in both precisions)") if fp64_batch else None,
+        "fp64_batched": fp64_batch, "fp64_plain": plain, "one_graph_launch": one_launch,
         "speedup_vs_fp64_plain_pair_evals": (value / plain["pair_evals_per_s"]) if plain else None,
         "roofline": roof,
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(probs)
-        log(f"[bench] cpu baseline {line['cpu_baseline']['value']:.3e} pair-evals/s")
+    if not args.no_cpu_baseline and infos64 is not None:
+        line["cpu_baseline"] = cpu_baseline(probs, infos64)
+        log(f"[bench] cpu baseline {line['cpu_baseline']['value']:.3e} pair-evals/s "
+            f"({line['cpu_baseline']['cores']} threads)")
     return line
 
 
@@ -495,7 +600,7 @@ def run_sphere(args):
     ev = sum(i["pair_evals"] for i in infos)
     t1 = time.perf_counter()
     errs, ev_cpu = [], 0.0
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     for q, lam in enumerate(wls):
         P = O.Problem(pos, probs[q]["polarisability"], probs[q]["k"], True, d)
         cso, _, io = P.compute_cross_sections(1e-10, args.max_iter, nthreads=threads)
@@ -659,7 +764,7 @@ def run_fields(args):
     e_near = float(np.max(np.abs(e[rows] - eo)) / np.max(np.abs(eo)))
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         per = 64
         chunks = [pts[i * per:(i + 1) * per] for i in range(threads)]
         with ThreadPoolExecutor(threads) as ex:
@@ -667,7 +772,7 @@ def run_fields(args):
             list(ex.map(lambda c: O.field_at_points(pos, p, k, c), chunks))
             dt = time.perf_counter() - t1
         cpu = {"value": per * threads * n / dt, "unit": "pair-evals/s", "cores": threads,
-               "kind": "port", "cpu_model": cpu_model(),
+               "kind": "port", **host_cpus(),
                "sample": f"oracle field_at_points, {threads} x {per} map points x {n} dipoles on "
                          f"{threads} threads in {dt:.2f} s"}
         log(f"[bench] fields cpu baseline {cpu['value']:.3e} pair-evals/s")
@@ -781,7 +886,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=64)
     ap.add_argument("--wavelengths", type=int, default=16)
-    ap.add_argument("--plain-count", type=int, default=64)
+    ap.add_argument("--plain-wavelengths", type=int, default=4)
+    ap.add_argument("--no-one-launch", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--no-error", action="store_true")

@@ -7,14 +7,12 @@
 
 namespace lh {
 
-// Whether solve s takes part in a launch: which = 0 solves inside the
-// Arnoldi loop, 1 solves that start a cycle, 2 solves normalising a new
-// basis vector, 3 every solve.
+// Whether solve s takes part in a launch (the kWhich codes of lh_types.h).
 __device__ __forceinline__ bool takes_part(const GState* st, int s, int which) {
-  if (which == 3) return true;
+  if (which == kWhichAll) return true;
   const GState& g = st[s];
-  if (which == 0) return g.inner != 0;
-  if (which == 1) return g.started != 0;
+  if (which == kWhichInner) return g.inner != 0;
+  if (which == kWhichStarted) return g.started != 0;
   return g.nflag != 0;
 }
 

@@ -160,6 +160,7 @@ class Engine {
   void record(hipEvent_t e);
   // adds the elapsed times of evs to the far-field sums (destroy: and frees them)
   void harvest(std::vector<EvPair>& evs, bool destroy);
+  void far_field(int p, int which, int in_slot);
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
   void arnoldi_step(int j, bool fp32);
   void enqueue_cycle(bool mixed, bool early_exit);

@@ -611,10 +611,14 @@ double Engine::pairs_per_matvec(int s) const {
 // Matvec and GMRES cycle.
 // ---------------------------------------------------------------------------
 
-void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
+// The far field of precision p (0 FP64, 1 FP32) for the solves `which`
+// selects: source records, then every far-field kernel of that precision
+// (and the FFT solves' block-Toeplitz products) into the partial slices,
+// bracketed by HIP events when timing.
+void Engine::far_field(int p, int which, int in_slot) {
   hipStream_t st = stream_;
+  const bool fp32 = p == 1;
   const int nv = (int)vit_.size();
-  const int p = fp32 ? 1 : 0;
   const int nfree = (int)far_free_[p].size(), nsub = (int)far_sub_[p].size();
   const bool rot = tiling_.block[p] == kSymRotBlock;
   const bool sym = tiling_.block[p] == kSymBlock || rot;
@@ -683,9 +687,15 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
              d_ewl_.as<double>(), d_part_.as<double2>(), blk);
   fft_matvec(in_slot, fp32);
   if (timing_) record(ev.b);
-  launch_reduce(st, nv, vi, sd, gs, which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
-                d_tab_.as<double>(), d_ainv_.as<double2>(), vec, in_slot, out_slot,
-                d_part_.as<double2>(), mode, d_norm_.as<double>());
+}
+
+// y = A x for the solves `which` selects.
+void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
+  far_field(fp32 ? 1 : 0, which, in_slot);
+  launch_reduce(stream_, (int)vit_.size(), d_vit_.as<VecItem>(), d_solves_.as<SolveDev>(),
+                d_state_.as<GState>(), which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
+                d_tab_.as<double>(), d_ainv_.as<double2>(), d_vec_.as<double2>(), in_slot,
+                out_slot, d_part_.as<double2>(), mode, d_norm_.as<double>());
 }
 
 void Engine::matvec_all(int in_slot, bool fp32) {

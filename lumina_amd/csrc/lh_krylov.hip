@@ -21,11 +21,7 @@
 namespace lh {
 
 __device__ __forceinline__ bool part_of(const GState* st, int s, int which) {
-  if (which == 3) return true;
-  const GState& g = st[s];
-  if (which == 0) return g.inner != 0;
-  if (which == 1) return g.started != 0;
-  return g.nflag != 0;
+  return takes_part(st, s, which);
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {

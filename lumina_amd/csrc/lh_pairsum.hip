@@ -71,12 +71,9 @@ __global__ __launch_bounds__(256) void k_near_tensors(const SolveDev* __restrict
   for (int q = 0; q < 12; ++q) o[q] = t[q];
 }
 
-// ---------------------------------------------------------------------------
-// Per-solve participation flags (which phase of the GMRES cycle uses a
-// kernel): 0 = in the Arnoldi loop, 1 = in a restart cycle, 2 = normalise
-// w -> v_{j+1}, 3 = always (single matvec).
-// ---------------------------------------------------------------------------
-
+// Every kernel below takes a participation code `which` (the kWhich codes
+// of lh_types.h): the phase of the GMRES cycle, and in mixed Arnoldi steps
+// the precision of the solve's products, that selects the solves it works on.
 
 // ---------------------------------------------------------------------------
 // Source records for one matvec: positions + input vector (+ image-scaled

@@ -41,6 +41,14 @@
 #define LH_TILE 256            // far-field tile: 256 targets x 256 sources
 #define LH_NDOT 32             // stride of the per-item dot partials
 
+// Participation codes of a launch (which solves a kernel works on): solves
+// in the Arnoldi loop, starting a cycle, normalising a new basis vector, all
+// of them.
+#define kWhichInner 0
+#define kWhichStarted 1
+#define kWhichNormalise 2
+#define kWhichAll 3
+
 struct SolveDev {
   int n;            // dipoles
   int m;            // restart length min(30, 3n)
