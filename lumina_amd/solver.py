@@ -444,7 +444,8 @@ class Batch:
 
     ``problems``: list of dicts with keys positions, polarisability, k,
     use_fcd, cell_size, incident (IncidentField), substrate (optional),
-    lattice (LatticeSpec, optional) and k_bloch (optional).
+    lattice (LatticeSpec, optional), k_bloch (optional) and
+    classify_positions (optional: positions of the FCD decision).
     Problems passing the same ``positions`` array object share its geometry.
     """
 
@@ -460,7 +461,8 @@ class Batch:
                 pos_cache[pid] = _f64(pr["positions"]).reshape(-1, 3)
             sp, kp = _spec(pos_cache[pid], pr["polarisability"], pr["k"], pr.get("use_fcd", True),
                            pr.get("cell_size", 3.0), pr.get("substrate"), FP64,
-                           pr.get("lattice"), pr.get("k_bloch", (0.0, 0.0, 0.0)))
+                           pr.get("lattice"), pr.get("k_bloch", (0.0, 0.0, 0.0)),
+                           pr.get("classify_positions"))
             keep.append(kp)
             arr[q].spec = sp
             arr[q].incident = pr.get("incident", IncidentField())._c()

@@ -199,15 +199,22 @@ def test_adjoint_substrate_and_unavailable(backend):
         sl.adjoint_grad(IncidentField(), 1e-10, 100, D, positions=False)
 
 
+@pytest.mark.timeout(900)
 def test_helix_full_size(backend):
     """The chiral helix at full size (coil 25 nm, pitch 40 nm, 4 turns, wire
-    8 nm, d=1.5: 35,013 dipoles) in water at 600 nm, LCP: the position
-    gradients of 8 seeded dipoles against the oracle's CPU FP64 adjoint on
-    the device's p and lam, and dC/dRe eps of 2 seeded dipoles against
-    central differences of the device FP64 C_ext."""
+    8 nm, d=1.5: 35,013 dipoles) in water at 600 nm, LCP, FP64 at tol 1e-12.
+    On 32 seeded dipoles: dC_ext/d eps and dC_ext/d r against the oracle's
+    CPU FP64 adjoint on the device's p and lam (1e-8), dC_ext/dRe eps against
+    central differences (relative step 1e-6 of |eps|) of the device FP64
+    C_ext, all 64 perturbed problems in one batch, and dC_ext/d r along a
+    seeded axis against central differences (step 1e-6 nm, the FCD
+    classification held at the unperturbed positions) on 8 of them (1e-5 of
+    the largest component)."""
+    from lumina_amd.solver import Batch, ConvergenceFailure
     lam, n_env, d = 600.0, 1.33, 1.5
     pos = I.au_helix_r25_p40_t4_w8_d1p5()
-    assert len(pos) == 35013
+    n = len(pos)
+    assert n == 35013
     k = I.wavenumber(lam, n_env)
     eps = I.material("Au").epsilon(lam)
     a = I.polarisability_block("Au", lam, d, n_env)
@@ -220,25 +227,46 @@ def test_helix_full_size(backend):
     _, p, _ = s.compute_cross_sections(inc, tol, mi)
     e = s.incident(inc)
     lv, _ = s.solve_gmres(np.conj(e), tol, mi)
-    rows = np.random.default_rng(1).choice(len(pos), 8, replace=False)
+    s.close()
+    rows = np.random.default_rng(1).choice(n, 32, replace=False)
     P = O.Problem(pos, a, k, True, d)
     geo, gro = P.adjoint_grad(p, lv, D, polarisation=LCP, rows=rows)
     e_pos = float(np.max(np.abs(gp[rows] - gro[rows])) / np.max(np.abs(gro[rows])))
-    e_eps = relmax(ge, geo)
-    print("helix adjoint vs oracle rows", e_pos, e_eps)
+    e_eps = float(np.max(np.abs(ge[rows] - geo[rows])) / np.max(np.abs(geo[rows])))
+    print(f"helix adjoint vs oracle on 32 dipoles: positions {e_pos:.1e}, eps {e_eps:.1e}")
     assert e_pos < 1e-8 and e_eps < 1e-8
-    alphas = np.tile(a, (len(pos), 1))
-    worst = 0.0
-    for i in np.random.default_rng(2).choice(len(pos), 2, replace=False):
-        h = 1e-6 * abs(eps)
-        c = []
+
+    # finite differences: one FP64 batch of every perturbed problem
+    h = 1e-6 * abs(eps)
+    hr = 1e-6
+    alphas = np.tile(a, (n, 1))
+    probs = []
+    for i in rows:
         for sg in (1, -1):
             a2 = alphas.copy()
             a2[i] = I.polarisability_from_epsilon(eps + sg * h, lam, d, n_env)
-            ss = PairSumSession(backend, pos, a2, k, True, d)
-            c.append(ss.compute_cross_sections(inc, tol, mi)[0][0])
-            ss.close()
-        fd = (c[0] - c[1]) / (2 * h)
-        worst = max(worst, abs(fd - ge[i, 0]) / np.max(np.abs(ge[:, 0])))
-        print("helix FD", i, fd, ge[i, 0])
-    assert worst < 1e-5
+            probs.append(dict(positions=pos, polarisability=a2, k=k, use_fcd=True, cell_size=d,
+                              incident=inc))
+    axes = np.random.default_rng(4).integers(0, 3, 8)
+    for i, ax in zip(rows[:8], axes):
+        for sg in (1, -1):
+            p2 = pos.copy()
+            p2[i, ax] += sg * hr
+            probs.append(dict(positions=p2, polarisability=a, k=k, use_fcd=True, cell_size=d,
+                              incident=inc, classify_positions=pos))
+    b = Batch(backend, probs)
+    b.run(tol, mi, FP64)
+    try:
+        c, infos, _ = b.results()
+    except ConvergenceFailure as ex:
+        raise AssertionError(f"finite-difference solve failed: {ex}")
+    b.close()
+    fd_eps = (c[0:64:2, 0] - c[1:64:2, 0]) / (2 * h)
+    err_eps = float(np.max(np.abs(fd_eps - ge[rows, 0])) / np.max(np.abs(ge[:, 0])))
+    fd_r = (c[64::2, 0] - c[65::2, 0]) / (2 * hr)
+    want_r = gp[rows[:8], axes]
+    err_r = float(np.max(np.abs(fd_r - want_r)) / np.max(np.abs(gp)))
+    print(f"helix FD: eps on 32 dipoles {err_eps:.1e}, r on 8 {err_r:.1e}; iterations "
+          f"{min(i['iterations'] for i in infos)}..{max(i['iterations'] for i in infos)}")
+    assert err_eps < 1e-5
+    assert err_r < 1e-5
