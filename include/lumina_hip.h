@@ -211,6 +211,18 @@ int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, s
 int lh_pairsum_field_at_points(lh_pairsum* s, const lh_c64* moments, const lh_incident* inc,
                                size_t m, const double* points, lh_c64* field,
                                double* intensity);
+/* No reference counterpart: lh_pairsum_field_at_points for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). moments (3N complex, or NULL for this session's last
+ * solution), points ([m][3]), field ([m][3] complex, may be NULL) and
+ * intensity ([m], may be NULL) are DEVICE pointers; the call allocates
+ * nothing: its working memory is a fixed 64 MiB array held by each GPU the
+ * library runs on until the process ends, which limits the session to
+ * N <= 262,144 dipoles (LH_INVALID_ARGUMENT beyond). Calls on one GPU share
+ * that array and are serialised. Returns when the results are written. */
+int lh_pairsum_field_at_points_noalloc(lh_pairsum* s, const lh_c64* moments,
+                                       const lh_incident* inc, size_t m, const double* points,
+                                       lh_c64* field, double* intensity);
 /* fields.rs:156-221 compute_far_field: theta = pi it / max(n_theta - 1, 1),
  * phi = 2 pi ip / n_phi (theta-major order), intensity = |(I - rhat rhat) F|^2
  * with F = sum_i p_i e^{-ik rhat.r_i}. moments as for

@@ -32,7 +32,8 @@ EXPORTS = [
     "lh_session_destroy",
     "lh_pairsum_create", "lh_pairsum_set_wavelength", "lh_pairsum_matvec", "lh_pairsum_solve",
     "lh_pairsum_incident", "lh_pairsum_cross_sections", "lh_pairsum_compute_cross_sections",
-    "lh_pairsum_adjoint_grad", "lh_pairsum_field_at_points", "lh_pairsum_far_field",
+    "lh_pairsum_adjoint_grad", "lh_pairsum_field_at_points",
+    "lh_pairsum_field_at_points_noalloc", "lh_pairsum_far_field",
     "lh_pairsum_destroy", "lh_batch_solve", "lh_batch_create", "lh_batch_run",
     "lh_batch_results", "lh_batch_destroy", "lh_batch_set_timing", "lh_batch_kernel_times",
     "lh_sweep_solve",
@@ -164,6 +165,9 @@ def load():
                                             C.c_int, D, D, D, C.POINTER(SolveInfo),
                                             C.POINTER(SolveInfo)]
     lib.lh_pairsum_field_at_points.argtypes = [P, P, C.POINTER(Incident), C.c_size_t, D, P, D]
+    # device pointers for moments, points, field and intensity
+    lib.lh_pairsum_field_at_points_noalloc.argtypes = [P, P, C.POINTER(Incident), C.c_size_t, P,
+                                                       P, P]
     lib.lh_pairsum_far_field.argtypes = [P, P, C.c_size_t, C.c_size_t, D, D, D]
     lib.lh_pairsum_destroy.argtypes = [P]
     lib.lh_pairsum_destroy.restype = None

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -223,9 +224,27 @@ void fill_info(const lh::SolveResult& r, lh_solve_info* info) {
 
 }  // namespace
 
+// Working memory of lh_pairsum_field_at_points_noalloc: a fixed array in
+// every GPU the library runs on, held until the process ends: the source
+// records of up to kNoAllocMaxDipoles dipoles and the chunk partials of the
+// point batches. Its name is a host-side stand-in, not a device address;
+// host code uses the address hipGetSymbolAddress returns on each device
+// (lh_backend::scratch), taken once per device when its backend is created.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoAllocMaxDipoles = 262144;   // 20 MiB of 80-byte source records
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+// Calls on one device share its scratch array: they are serialised per
+// device (ordinals beyond the table share a lock).
+constexpr int kLockedDevices = 64;
+std::mutex g_scratch_lock[kLockedDevices];
+}  // namespace
+
 struct lh_backend {
   int device = 0;
   hipDeviceProp_t prop;
+  void* scratch = nullptr;   // this device's copy of `scratch`
 };
 
 struct lh_session {
@@ -280,6 +299,8 @@ int lh_backend_create(int device_ordinal, lh_backend** out) {
     b->device = device_ordinal;
     LH_HIP(hipSetDevice(device_ordinal));
     LH_HIP(hipGetDeviceProperties(&b->prop, device_ordinal));
+    // the device address of this device's scratch array (with the device current)
+    LH_HIP(hipGetSymbolAddress(&b->scratch, HIP_SYMBOL(scratch)));
     *out = b.release();
     return (int)LH_OK;
   });
@@ -556,6 +577,29 @@ int lh_pairsum_field_at_points(lh_pairsum* s, const lh_c64* moments, const lh_in
     LH_HIP(hipSetDevice(s->b->device));
     const int slot = moments_slot(s, moments);
     s->eng->field_at_points(0, slot, *inc, (long long)m, points, field, intensity);
+    return (int)LH_OK;
+  });
+}
+
+int lh_pairsum_field_at_points_noalloc(lh_pairsum* s, const lh_c64* moments,
+                                       const lh_incident* inc, size_t m, const double* points,
+                                       lh_c64* field, double* intensity) {
+  return guarded([&]() {
+    if (!s || !inc || (m > 0 && !points)) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (m > kMaxFieldPoints) return fail(LH_INVALID_ARGUMENT, "too many observation points");
+    if (s->n > kNoAllocMaxDipoles)
+      return fail(LH_INVALID_ARGUMENT,
+                  "lh_pairsum_field_at_points_noalloc: " + std::to_string(s->n) +
+                      " dipoles exceed the fixed scratch's " +
+                      std::to_string(kNoAllocMaxDipoles));
+    if (m == 0 || (!field && !intensity)) return (int)LH_OK;
+    if (!s->b->scratch) return fail(LH_UNAVAILABLE, "no scratch address on this device");
+    LH_HIP(hipSetDevice(s->b->device));
+    std::lock_guard<std::mutex> lock(g_scratch_lock[s->b->device % kLockedDevices]);
+    s->eng->field_at_points_scratch(0, LH_SLOT_X, reinterpret_cast<const double2*>(moments), *inc,
+                                    (long long)m, points, reinterpret_cast<double2*>(field),
+                                    intensity, static_cast<char*>(s->b->scratch),
+                                    kNoAllocScratchBytes);
     return (int)LH_OK;
   });
 }

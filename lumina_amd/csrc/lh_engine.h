@@ -121,6 +121,14 @@ class Engine {
   // may be null) and |E|^2 (m, may be null) to the host.
   void field_at_points(int s, int slot, const lh_incident& inc, long long m, const double* pts,
                        lh_c64* field, double* intensity);
+  // field_at_points on device buffers (points 3m doubles, field 3m complex
+  // and intensity m, each may be null) from the moments mom (3n complex,
+  // device; null: slot `slot`), with `scratch` (device, bytes long) as the
+  // only working memory: allocates nothing.
+  void field_at_points_scratch(int s, int slot, const double2* mom, const lh_incident& inc,
+                               long long m, const double* pts, double2* field, double* intensity,
+                               char* scratch, size_t bytes);
+  const SolveDev& solve_dev(int s) const { return sd_[s]; }
   // fields.rs:156-221: |(I - rhat rhat) F|^2 for m directions (host, 3m
   // doubles) from the moments in slot `slot` of solve s.
   void far_field(int s, int slot, long long m, const double* rhat, double* intensity);
@@ -205,7 +213,7 @@ class Engine {
   DevMem d_gpack_, d_gizj2_, d_ewl_, d_ewt64_, d_ewt32_, d_ewi_s_, d_ewi_e_;
   DevMem d_gtab_, d_dainv_, d_gout_, d_far_grad_;   // adjoint: near gradient tensors, D
                                                      // blocks, output, far items
-  DevMem d_fsrc_, d_fpts_, d_fpart_, d_fout_, d_fint_, d_fpol_;   // field maps
+  DevMem d_fsrc_, d_fpts_, d_fpart_, d_fout_, d_fint_;   // field maps
   void field_sources(int s, int slot);
   // FFT block-Toeplitz matvec of one full-grid solve (lh_fft.hip)
   struct FftSolve {

@@ -1186,12 +1186,11 @@ void Engine::field_at_points(int s, int slot, const lh_incident& inc, long long 
   const SolveDev& d = sd_[s];
   hipStream_t st = stream_;
   field_sources(s, slot);
-  std::vector<double> pol(6);
+  double pol[6];
   for (int c = 0; c < 3; ++c) {
     pol[2 * c] = inc.polarisation[c].re;
     pol[2 * c + 1] = inc.polarisation[c].im;
   }
-  upload(d_fpol_, pol, st);
   for (long long q0 = 0; q0 < m; q0 += kFieldBatch) {
     const int mb = (int)std::min(kFieldBatch, m - q0);
     const int chunk = field_chunk(mb, d.n);
@@ -1205,8 +1204,8 @@ void Engine::field_at_points(int s, int slot, const lh_incident& inc, long long 
     launch_field_part(st, mb, d_fpts_.as<double>(), d_fsrc_.as<FieldSrc>(), d.n, chunk, d.k,
                       d_fpart_.as<double2>());
     launch_field_reduce(st, mb, nchunk, d_fpts_.as<double>(), d_fpart_.as<double2>(), d.k,
-                        inc.direction, inc.amplitude, d_fpol_.as<double>(),
-                        d_fout_.as<double2>(), d_fint_.as<double>());
+                        inc.direction, inc.amplitude, pol, d_fout_.as<double2>(),
+                        d_fint_.as<double>());
     LH_HIP(hipGetLastError());
     if (field)
       LH_HIP(hipMemcpyAsync(field + 3 * q0, d_fout_.p, sizeof(double2) * 3 * (size_t)mb,
@@ -1216,6 +1215,51 @@ void Engine::field_at_points(int s, int slot, const lh_incident& inc, long long 
                             hipMemcpyDeviceToHost, st));
     LH_HIP(hipStreamSynchronize(st));
   }
+}
+
+// field_at_points with every buffer on the device and the scratch memory
+// given: the source records and the chunk partials live in `scratch`
+// (`bytes` long, 256-byte aligned); nothing is allocated. mom: the moments
+// (3n complex, device) or null for slot `slot`. The point batches are sized
+// so that their partials fit the scratch; the caller has checked that the
+// source records do.
+void Engine::field_at_points_scratch(int s, int slot, const double2* mom, const lh_incident& inc,
+                                     long long m, const double* pts, double2* field,
+                                     double* intensity, char* scratch, size_t bytes) {
+  const SolveDev& d = sd_[s];
+  hipStream_t st = stream_;
+  const size_t src_bytes = (sizeof(FieldSrc) * (size_t)d.n + 255) / 256 * 256;
+  FieldSrc* src = reinterpret_cast<FieldSrc*>(scratch);
+  double2* part = reinterpret_cast<double2*>(scratch + src_bytes);
+  const size_t part_cap = (bytes - src_bytes) / (3 * sizeof(double2));   // points x chunks
+  launch_field_pack(st, d.n, d_px_.as<double>() + d.pos_off, d_py_.as<double>() + d.pos_off,
+                    d_pz_.as<double>() + d.pos_off,
+                    mom ? mom : d_vec_.as<double2>() + d.vec_off + (long long)slot * 3 * d.n,
+                    src);
+  double pol[6];
+  for (int c = 0; c < 3; ++c) {
+    pol[2 * c] = inc.polarisation[c].re;
+    pol[2 * c + 1] = inc.polarisation[c].im;
+  }
+  for (long long q0 = 0; q0 < m;) {
+    // the batch of field_at_points, halved until its partials fit
+    int mb = (int)std::min(kFieldBatch, m - q0);
+    int chunk = field_chunk(mb, d.n), nchunk = (d.n + chunk - 1) / chunk;
+    while ((size_t)mb * nchunk > part_cap && mb > 256) {
+      mb = std::max(256, mb / 2);
+      chunk = field_chunk(mb, d.n);
+      nchunk = (d.n + chunk - 1) / chunk;
+    }
+    if ((size_t)mb * nchunk > part_cap)
+      throw Error(LH_INVALID_ARGUMENT, "field map scratch too small for this dipole count");
+    launch_field_part(st, mb, pts + 3 * q0, src, d.n, chunk, d.k, part);
+    launch_field_reduce(st, mb, nchunk, pts + 3 * q0, part, d.k, inc.direction, inc.amplitude,
+                        pol, field ? field + 3 * q0 : nullptr,
+                        intensity ? intensity + q0 : nullptr);
+    LH_HIP(hipGetLastError());
+    q0 += mb;
+  }
+  LH_HIP(hipStreamSynchronize(st));
 }
 
 void Engine::far_field(int s, int slot, long long m, const double* rhat, double* intensity) {

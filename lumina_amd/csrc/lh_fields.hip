@@ -70,11 +70,17 @@ __global__ __launch_bounds__(256) void k_field_part(int m, const double* __restr
   }
 }
 
+// The incident polarisation (re, im per component), passed by value.
+struct Pol6 {
+  double v[6];
+};
+
+// field and intensity may be null (not written).
 __global__ __launch_bounds__(256) void k_field_reduce(int m, int nchunk,
                                                       const double* __restrict__ pts,
                                                       const double2* __restrict__ part, double k,
                                                       double dx, double dy, double dz,
-                                                      double amp, const double* __restrict__ pol,
+                                                      double amp, Pol6 pol,
                                                       double2* __restrict__ field,
                                                       double* __restrict__ intensity) {
   const int q = blockIdx.x * 256 + threadIdx.x;
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(256) void k_field_reduce(int m, int nchunk,
   sincos_f64(kdotr, sn, cs);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double er = amp * pol[2 * c], ei = amp * pol[2 * c + 1];
+    const double er = amp * pol.v[2 * c], ei = amp * pol.v[2 * c + 1];
     e[2 * c] = er * cs - ei * sn;
     e[2 * c + 1] = er * sn + ei * cs;
   }
@@ -101,10 +107,10 @@ __global__ __launch_bounds__(256) void k_field_reduce(int m, int nchunk,
   double in = 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    field[3LL * q + c] = make_double2(e[2 * c], e[2 * c + 1]);
+    if (field) field[3LL * q + c] = make_double2(e[2 * c], e[2 * c + 1]);
     in += e[2 * c] * e[2 * c] + e[2 * c + 1] * e[2 * c + 1];
   }
-  intensity[q] = in;
+  if (intensity) intensity[q] = in;
 }
 
 __global__ __launch_bounds__(256) void k_ff_part(int m, const double* __restrict__ rhat,
@@ -183,10 +189,12 @@ void launch_field_part(hipStream_t st, int m, const double* pts, const FieldSrc*
 
 void launch_field_reduce(hipStream_t st, int m, int nchunk, const double* pts,
                          const double2* part, double k, const double dir[3], double amp,
-                         const double* pol, double2* field, double* intensity) {
+                         const double pol[6], double2* field, double* intensity) {
   if (m <= 0) return;
+  Pol6 p6;
+  for (int q = 0; q < 6; ++q) p6.v[q] = pol[q];
   hipLaunchKernelGGL(k_field_reduce, dim3((m + 255) / 256), dim3(256), 0, st, m, nchunk, pts, part,
-                     k, dir[0], dir[1], dir[2], amp, pol, field, intensity);
+                     k, dir[0], dir[1], dir[2], amp, p6, field, intensity);
 }
 
 void launch_ff_part(hipStream_t st, int m, const double* rhat, const FieldSrc* src, int n, int chunk,
