@@ -16,6 +16,22 @@ __device__ __forceinline__ bool takes_part(const GState* st, int s, int which) {
   return g.nflag != 0;
 }
 
+// One far-field partial (re, im) at complex index e of the partial slices:
+// FP64, or rounded to FP32 when p32 (the slices of an FP32 matvec, which
+// then occupy the first half of the buffer as float2 and are read back so).
+__device__ __forceinline__ void put_part(double2* part, long long e, double re, double im,
+                                         int p32) {
+  if (p32) reinterpret_cast<float2*>(part)[e] = make_float2((float)re, (float)im);
+  else part[e] = make_double2(re, im);
+}
+__device__ __forceinline__ double2 get_part(const double2* part, long long e, int p32) {
+  if (p32) {
+    const float2 v = reinterpret_cast<const float2*>(part)[e];
+    return make_double2((double)v.x, (double)v.y);
+  }
+  return part[e];
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -53,6 +53,7 @@ struct Tiling {
                               // dipoles, 2 = from 2, 0 = never
   int cgs2_fp32 = 0;          // Arnoldi steps with an FP32 matvec: classical Gram-Schmidt
                               // with reorthogonalisation (1) or one classical pass (0)
+  int part32 = 1;             // FP32 matvecs store their far partial slices in FP32
   int graph_cycles = 0;       // restart cycles per graph launch: 0 = 1, 2, 4, ... up to
                               // kAutoGraphCycles; N > 0 = N; -1 = enough cycles to reach
                               // max_iter in one launch (at most kMaxGraphCycles)
@@ -169,6 +170,7 @@ class Engine {
   // adds the elapsed times of evs to the far-field sums (destroy: and frees them)
   void harvest(std::vector<EvPair>& evs, bool destroy);
   void far_field(int p, int which, int in_slot);
+  int part32(bool fp32) const { return (fp32 && tiling_.part32) ? 1 : 0; }
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
   void arnoldi_step(int j, bool fp32);
   void enqueue_cycle(bool mixed, bool early_exit);

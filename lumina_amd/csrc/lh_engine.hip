@@ -21,20 +21,20 @@ void launch_prep32(hipStream_t, int, const VecItem*, const SolveDev*, const GSta
 void launch_far64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                   const Src64*, double2*, int);
 void launch_far32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                  const Src32*, double2*, bool, int);
+                  const Src32*, double2*, bool, int, int);
 void launch_far_clean64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                         const Src64*, double2*, int);
 void launch_far_clean32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
-                        const Src32*, double2*, bool, int);
+                        const Src32*, double2*, bool, int, int);
 void launch_far_sym32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                      const Src32*, double2*);
+                      const Src32*, double2*, int);
 void launch_far_sym64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
                       const Src64*, double2*);
 void launch_far_symrot64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                          const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
-                   const double2*, int, double*);
+                   const double2*, int, int, double*);
 void launch_vnorm(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                   const double2*, int, double*);
 void launch_scale(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
@@ -56,9 +56,9 @@ void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int
 void launch_ewald_table(hipStream_t, int, const SolveDev*, const int*, const long long*,
                         const double*, double*, float*);
 void launch_ewp(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
-                const Src64*, const double*, double2*, int);
+                const Src64*, const double*, double2*, int, int);
 void launch_ewt_shift32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
-                        const Src32*, const int*, const int*, const float*, double2*);
+                        const Src32*, const int*, const int*, const float*, double2*, int);
 void launch_ewt_shift64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                         const Src64*, const int*, const int*, const double*, double2*);
 
@@ -78,7 +78,8 @@ void launch_fft_scatter32(hipStream_t, long long, const int*, const double2*, fl
 void launch_fft_mul64(hipStream_t, long long, const double2*, double2*);
 void launch_fft_mul32(hipStream_t, long long, const float2*, float2*);
 void launch_fft_gather64(hipStream_t, int, long long, const int*, const double2*, double, double2*);
-void launch_fft_gather32(hipStream_t, int, long long, const int*, const float2*, double, double2*);
+void launch_fft_gather32(hipStream_t, int, long long, const int*, const float2*, double, double2*,
+                         int);
 void launch_sweep_setup(hipStream_t, const SweepDevArgs&, const SolveDev*, int, double2*);
 void launch_sweep_incident(hipStream_t, const SolveDev*, int, int, const double*, const double*,
                            const double*, const double[3], double, const double*, double2*);
@@ -634,6 +635,8 @@ void Engine::far_field(int p, int which, int in_slot) {
     (cap_evs_ ? *cap_evs_ : evs_).push_back(ev);
   }
   const int newt = (int)far_ewt_[p].size(), newp = (int)far_ewp_[p].size();
+  // an FP32 matvec keeps its partial slices in FP32
+  const int p32 = part32(fp32);
   if (fp32) {
     launch_prep32(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src32_.as<Src32>());
@@ -644,21 +647,21 @@ void Engine::far_field(int p, int which, int in_slot) {
     if (timing_) record(ev.a);
     if (sym) {
       launch_far_sym32(st, true, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
-                       d_src32_.as<Src32>(), d_part_.as<double2>());
+                       d_src32_.as<Src32>(), d_part_.as<double2>(), p32);
       launch_far_sym32(st, false, (int)far_clean_[1].size(), d_far_clean_[1].as<FarItem>(), sd,
-                       gs, which, d_src32_.as<Src32>(), d_part_.as<double2>());
+                       gs, which, d_src32_.as<Src32>(), d_part_.as<double2>(), p32);
     } else {
       launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
-                   d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+                   d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk, p32);
       launch_far_clean32(st, (int)far_clean_[1].size(), d_far_clean_[1].as<FarItem>(), sd, gs,
                          which, d_src32_.as<Src32>(), d_part_.as<double2>(),
-                         tiling_.hw_sincos != 0, blk);
+                         tiling_.hw_sincos != 0, blk, p32);
     }
     launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
-                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk);
+                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk, p32);
     launch_ewt_shift32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which,
                        d_src32_.as<Src32>(), d_gpack_.as<int>(), d_gizj2_.as<int>(),
-                       d_ewt32_.as<float>(), d_part_.as<double2>());
+                       d_ewt32_.as<float>(), d_part_.as<double2>(), p32);
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
@@ -684,7 +687,7 @@ void Engine::far_field(int p, int which, int in_slot) {
                        d_ewt64_.as<double>(), d_part_.as<double2>());
   }
   launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
-             d_ewl_.as<double>(), d_part_.as<double2>(), blk);
+             d_ewl_.as<double>(), d_part_.as<double2>(), blk, p32);
   fft_matvec(in_slot, fp32);
   if (timing_) record(ev.b);
 }
@@ -695,7 +698,7 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   launch_reduce(stream_, (int)vit_.size(), d_vit_.as<VecItem>(), d_solves_.as<SolveDev>(),
                 d_state_.as<GState>(), which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
                 d_tab_.as<double>(), d_ainv_.as<double2>(), d_vec_.as<double2>(), in_slot,
-                out_slot, d_part_.as<double2>(), mode, d_norm_.as<double>());
+                out_slot, d_part_.as<double2>(), part32(fp32), mode, d_norm_.as<double>());
 }
 
 void Engine::matvec_all(int in_slot, bool fp32) {
@@ -1139,7 +1142,8 @@ void Engine::fft_matvec(int in_slot, bool fp32) {
       check_fft(hipfftExecC2C(f.plan32, X, X, HIPFFT_FORWARD), "exec C2C");
       launch_fft_mul32(st, f.len, f.K32.as<float2>(), f.X32.as<float2>());
       check_fft(hipfftExecC2C(f.plan32, X, X, HIPFFT_BACKWARD), "exec C2C");
-      launch_fft_gather32(st, d.n, f.len, f.dip2pt.as<int>(), f.X32.as<float2>(), scale, part);
+      launch_fft_gather32(st, d.n, f.len, f.dip2pt.as<int>(), f.X32.as<float2>(), scale, part,
+                          part32(true));
     } else {
       hipfftDoubleComplex* X = f.X64.as<hipfftDoubleComplex>();
       launch_fft_scatter64(st, f.len, f.pt2dip.as<int>(), x, f.X64.as<double2>());

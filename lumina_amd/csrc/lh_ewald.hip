@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64) void k_ewt_shift(const FarItem* __restrict__ it
                                                  const int* __restrict__ gpack,
                                                  const int* __restrict__ gizj2,
                                                  const R* __restrict__ tab,
-                                                 double2* __restrict__ part) {
+                                                 double2* __restrict__ part, int p32) {
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
@@ -283,11 +283,11 @@ __global__ __launch_bounds__(64) void k_ewt_shift(const FarItem* __restrict__ it
     }
   }
   if (valid) {
-    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
+    const long long e = S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      o[c] = make_double2((double)hi[2 * c] + (double)lo[2 * c],
-                          (double)hi[2 * c + 1] + (double)lo[2 * c + 1]);
+      put_part(part, e + c, (double)hi[2 * c] + (double)lo[2 * c],
+               (double)hi[2 * c + 1] + (double)lo[2 * c + 1], p32);
   }
 }
 
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_ewp_row(const FarItem* __restrict__ ite
                                                  const GState* __restrict__ st, int which,
                                                  const Src64* __restrict__ src,
                                                  const double* __restrict__ ewl,
-                                                 double2* __restrict__ part) {
+                                                 double2* __restrict__ part, int p32) {
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
@@ -324,9 +324,9 @@ __global__ __launch_bounds__(256) void k_ewp_row(const FarItem* __restrict__ ite
     }
   }
   if (valid) {
-    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
+    const long long e = S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = make_double2(acc[2 * c], acc[2 * c + 1]);
+    for (int c = 0; c < 3; ++c) put_part(part, e + c, acc[2 * c], acc[2 * c + 1], p32);
   }
 }
 
@@ -343,10 +343,10 @@ void launch_ewald_table(hipStream_t st, int nitems, const SolveDev* solves, cons
 
 void launch_ewt_shift32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
                         const GState* gs, int which, const Src32* src, const int* gpack,
-                        const int* gizj2, const float* tab, double2* part) {
+                        const int* gizj2, const float* tab, double2* part, int p32) {
   if (nitems <= 0) return;
   hipLaunchKernelGGL((k_ewt_shift<float, Src32>), dim3(nitems), dim3(64), 0, st, items, solves,
-                     gs, which, src, gpack, gizj2, tab, part);
+                     gs, which, src, gpack, gizj2, tab, part, p32);
 }
 
 void launch_ewt_shift64(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
@@ -354,15 +354,15 @@ void launch_ewt_shift64(hipStream_t st, int nitems, const FarItem* items, const 
                         const int* gizj2, const double* tab, double2* part) {
   if (nitems <= 0) return;
   hipLaunchKernelGGL((k_ewt_shift<double, Src64>), dim3(nitems), dim3(64), 0, st, items, solves,
-                     gs, which, src, gpack, gizj2, tab, part);
+                     gs, which, src, gpack, gizj2, tab, part, 0);
 }
 
 void launch_ewp(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
                 const GState* gs, int which, const Src64* src, const double* ewl, double2* part,
-                int block) {
+                int block, int p32) {
   if (nitems <= 0) return;
   hipLaunchKernelGGL(k_ewp_row, dim3(nitems), dim3(block), 0, st, items, solves, gs, which, src,
-                     ewl, part);
+                     ewl, part, p32);
 }
 
 }  // namespace lh

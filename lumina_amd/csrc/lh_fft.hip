@@ -139,14 +139,14 @@ template <class T2>
 __global__ __launch_bounds__(256) void k_fft_gather(int n, long long len,
                                                     const int* __restrict__ dip2pt,
                                                     const T2* __restrict__ X, double scale,
-                                                    double2* __restrict__ part) {
+                                                    double2* __restrict__ part, int p32) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const long long q = dip2pt[i];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const T2 v = X[c * len + q];
-    part[3LL * i + c] = make_double2((double)v.x * scale, (double)v.y * scale);
+    put_part(part, 3LL * i + c, (double)v.x * scale, (double)v.y * scale, p32);
   }
 }
 
@@ -194,13 +194,13 @@ void launch_fft_mul32(hipStream_t st, long long len, const float2* K, float2* X)
 void launch_fft_gather64(hipStream_t st, int n, long long len, const int* dip2pt,
                          const double2* X, double scale, double2* part) {
   hipLaunchKernelGGL(k_fft_gather<double2>, grid_of(n), dim3(256), 0, st, n, len, dip2pt, X,
-                     scale, part);
+                     scale, part, 0);
 }
 
 void launch_fft_gather32(hipStream_t st, int n, long long len, const int* dip2pt,
-                         const float2* X, double scale, double2* part) {
+                         const float2* X, double scale, double2* part, int p32) {
   hipLaunchKernelGGL(k_fft_gather<float2>, grid_of(n), dim3(256), 0, st, n, len, dip2pt, X,
-                     scale, part);
+                     scale, part, p32);
 }
 
 }  // namespace lh

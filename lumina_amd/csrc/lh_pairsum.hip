@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
                                                  const SolveDev* __restrict__ solves,
                                                  const GState* __restrict__ st, int which,
                                                  const typename SrcOf<R>::type* __restrict__ src,
-                                                 double2* __restrict__ part) {
+                                                 double2* __restrict__ part, int p32) {
   typedef typename SrcOf<R>::type SRC;
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
@@ -313,10 +313,10 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
     row.fold(acc);
   }
   if (valid) {
-    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
+    const long long e = S.part_off + (long long)(it.c0 / LH_TILE) * 3LL * S.n + 3LL * i;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      o[c] = make_double2(row.get(2 * c) * S.c0, row.get(2 * c + 1) * S.c0);
+      put_part(part, e + c, row.get(2 * c) * S.c0, row.get(2 * c + 1) * S.c0, p32);
   }
 }
 
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
                                                    const SolveDev* __restrict__ solves,
                                                    const GState* __restrict__ st, int which,
                                                    const Src32* __restrict__ src,
-                                                   double2* __restrict__ part) {
+                                                   double2* __restrict__ part, int p32) {
   __shared__ float4 s_a[128];        // x, y, z, v0 of source (e mod 64) of the sub-block
   __shared__ float4 s_b[128];        // v1 .. v4
   __shared__ float s_c[128];         // v5
@@ -495,20 +495,21 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
         for (int q = 0; q < 6; ++q) tr[q] = t[q];
       } else if (jb + lane < S.n) {
         // transposed sums of source jb + lane go to the target tile's slice
-        double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
+        const long long e =
+            S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o[c] = make_double2(t[2 * c] * c0, t[2 * c + 1] * c0);
+        for (int c = 0; c < 3; ++c) put_part(part, e + c, t[2 * c] * c0, t[2 * c + 1] * c0, p32);
       }
     }
   }
   if (tvalid) {
     // forward sums (plus, on the diagonal, the transposed sums of the same
     // dipole) to the source tile's slice
-    double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+    const long long e = S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      o[c] = make_double2((((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
-                          (((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]) * c0);
+      put_part(part, e + c, (((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
+               (((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]) * c0, p32);
   }
 }
 
@@ -866,7 +867,7 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
     const GState* __restrict__ st, int which, const int* __restrict__ ell_col,
     const int* __restrict__ ell_tid, const double* __restrict__ tab,
     const double2* __restrict__ ainv, double2* __restrict__ vec, int in_slot, int out_slot,
-    const double2* __restrict__ part, int mode, double* __restrict__ norm_part) {
+    const double2* __restrict__ part, int p32, int mode, double* __restrict__ norm_part) {
   __shared__ double lds[4];
   const VecItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
@@ -905,10 +906,10 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
     // solve, whose 151 blocks leave CUs idle.
     const int nslice = S.fft ? 1 : S.nt;
     for (int ch = 0; ch < nslice; ++ch) {
-      const double2* p = part + S.part_off + (long long)ch * n3 + 3LL * i;
+      const long long e = S.part_off + (long long)ch * n3 + 3LL * i;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double2 q = p[c];
+        const double2 q = get_part(part, e + c, p32);
         yr[c] -= q.x;
         yi[c] -= q.y;
       }
@@ -995,28 +996,28 @@ void launch_far64(hipStream_t st, bool sub, int nitems, const FarItem* items,
   if (nitems <= 0) return;
   if (sub)
     hipLaunchKernelGGL((k_far_row<double, true, false>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, 0);
   else
     hipLaunchKernelGGL((k_far_row<double, false, false>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, 0);
 }
 
 void launch_far32(hipStream_t st, bool sub, int nitems, const FarItem* items,
                   const SolveDev* solves, const GState* gs, int which, const Src32* src,
-                  double2* part, bool hw_sincos, int block) {
+                  double2* part, bool hw_sincos, int block, int p32) {
   if (nitems <= 0) return;
   if (sub && hw_sincos)
     hipLaunchKernelGGL((k_far_row<float, true, true>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, p32);
   else if (sub)
     hipLaunchKernelGGL((k_far_row<float, true, false>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, p32);
   else if (hw_sincos)
     hipLaunchKernelGGL((k_far_row<float, false, true>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, p32);
   else
     hipLaunchKernelGGL((k_far_row<float, false, false>), dim3(nitems), dim3(block), 0, st, items,
-                       solves, gs, which, src, part);
+                       solves, gs, which, src, part, p32);
 }
 
 // Free-space items without excluded pairs (k_far_row<R, false, HW, false>).
@@ -1024,32 +1025,32 @@ void launch_far_clean64(hipStream_t st, int nitems, const FarItem* items, const 
                         const GState* gs, int which, const Src64* src, double2* part, int block) {
   if (nitems <= 0) return;
   hipLaunchKernelGGL((k_far_row<double, false, false, false>), dim3(nitems), dim3(block), 0, st,
-                     items, solves, gs, which, src, part);
+                     items, solves, gs, which, src, part, 0);
 }
 
 void launch_far_clean32(hipStream_t st, int nitems, const FarItem* items, const SolveDev* solves,
                         const GState* gs, int which, const Src32* src, double2* part,
-                        bool hw_sincos, int block) {
+                        bool hw_sincos, int block, int p32) {
   if (nitems <= 0) return;
   if (hw_sincos)
     hipLaunchKernelGGL((k_far_row<float, false, true, false>), dim3(nitems), dim3(block), 0, st,
-                       items, solves, gs, which, src, part);
+                       items, solves, gs, which, src, part, p32);
   else
     hipLaunchKernelGGL((k_far_row<float, false, false, false>), dim3(nitems), dim3(block), 0, st,
-                       items, solves, gs, which, src, part);
+                       items, solves, gs, which, src, part, p32);
 }
 
 // near = false: items without excluded pairs (k_far_sym32<false>).
 void launch_far_sym32(hipStream_t st, bool near, int nitems, const FarItem* items,
                       const SolveDev* solves, const GState* gs, int which, const Src32* src,
-                      double2* part) {
+                      double2* part, int p32) {
   if (nitems <= 0) return;
   if (near)
     hipLaunchKernelGGL((k_far_sym32<true>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
-                       which, src, part);
+                       which, src, part, p32);
   else
     hipLaunchKernelGGL((k_far_sym32<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
-                       which, src, part);
+                       which, src, part, p32);
 }
 
 // The rotating form (k_far_sym<double>), every item T >= S with the
@@ -1077,9 +1078,10 @@ void launch_far_sym64(hipStream_t st, bool near, int nitems, const FarItem* item
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
                    const GState* gs, int which, const int* ell_col, const int* ell_tid,
                    const double* tab, const double2* ainv, double2* vec, int in_slot,
-                   int out_slot, const double2* part, int mode, double* norm_part) {
+                   int out_slot, const double2* part, int p32, int mode, double* norm_part) {
   hipLaunchKernelGGL(k_reduce, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, which,
-                     ell_col, ell_tid, tab, ainv, vec, in_slot, out_slot, part, mode, norm_part);
+                     ell_col, ell_tid, tab, ainv, vec, in_slot, out_slot, part, p32, mode,
+                     norm_part);
 }
 
 }  // namespace lh
