@@ -273,7 +273,7 @@ def run_batch(args):
         blocks = (kt["far64_block"], kt["far32_block"], kt["tune_ms"])
         log(f"[bench] step {s_}: {dt:.2f} s, iters mean "
             f"{np.mean([i['iterations'] for i in infos]):.1f}, max "
-            f"{max(i['iterations'] for i in infos)}, {kt['host_round_trips']} graph launches "
+            f"{max(i['iterations'] for i in infos)}, {kt['host_round_trips']} host round trips "
             f"({kt['cycles_issued']} cycles)")
     total = sum(times)
     value = evals / total
@@ -303,7 +303,7 @@ def run_batch(args):
                       "cycles_issued": kt1["cycles_issued"],
                       "identical_to_timed_path": bool(np.array_equal(cs1, cs_mixed))}
         log(f"[bench] one graph launch per solve: {dt1:.2f} s, "
-            f"{kt1['host_round_trips']} launch(es), same results: "
+            f"{kt1['host_round_trips']} host round trip(s), same results: "
             f"{one_launch['identical_to_timed_path']}")
 
     # FP64 answer of every solve, one FP64 batch at the same tolerance (the

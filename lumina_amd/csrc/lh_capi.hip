@@ -189,6 +189,8 @@ lh::Tiling tiling_from_env() {
   const char* ff = std::getenv("LUMINA_FFT");
   if (ff && std::string(ff) == "0") t.fft = 0;
   if (ff && std::string(ff) == "force") t.fft = 2;
+  const char* og = std::getenv("LUMINA_ORTHO");
+  if (og && std::string(og) == "mgs") t.mgs = 1;
   const char* pp = std::getenv("LUMINA_PART64");
   if (pp && std::string(pp) == "1") t.part32 = 0;
   const char* gc = std::getenv("LUMINA_GRAPH_CYCLES");

@@ -53,6 +53,8 @@ struct Tiling {
                               // dipoles, 2 = from 2, 0 = never
   int cgs2_fp32 = 0;          // Arnoldi steps with an FP32 matvec: classical Gram-Schmidt
                               // with reorthogonalisation (1) or one classical pass (0)
+  int mgs = 0;                // FP64 Arnoldi steps: modified Gram-Schmidt in the reference's
+                              // order (1) instead of CGS2 (0)
   int part32 = 1;             // FP32 matvecs store their far partial slices in FP32
   int graph_cycles = 0;       // restart cycles per graph launch: 0 = 1, 2, 4, ... up to
                               // kAutoGraphCycles; N > 0 = N; -1 = enough cycles to reach

@@ -40,15 +40,16 @@ void launch_vnorm(hipStream_t, int, const VecItem*, const SolveDev*, const GStat
 void launch_scale(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                   double2*, int, int);
 void launch_dots(hipStream_t, int, const VecItem*, const SolveDev*, const GState*,
-                 const double2*, int, double2*);
+                 const double2*, int, int, double2*);
 void launch_update(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, double2*,
-                   int, double*);
+                   int, int, double*);
 void launch_xupdate(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, double2*);
 void launch_cross(hipStream_t, int, const VecItem*, const SolveDev*, const double2*,
                   const double2*, double*);
 void launch_copy_slot(hipStream_t, int, const VecItem*, const SolveDev*, double2*, int, int);
 void launch_zero(hipStream_t, int*);
-void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, const double2*, int, int*);
+void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, int, const double2*, int,
+                     int*);
 void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const double*, int, int*);
 void launch_backsub(hipStream_t, int, GState*);
 void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*);
@@ -722,20 +723,32 @@ void Engine::arnoldi_step(int j, bool fp32) {
   GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
   matvec(0, j, LH_SLOT_W, 0, fp32);
-  // FP64 steps: classical Gram-Schmidt with one reorthogonalisation pass
-  // (CGS2), orthogonal to working precision like the reference's MGS
-  // (iterative.rs:92-96). Steps with an FP32 matvec take one classical pass
-  // (FP64 arithmetic): the basis only has to be as good as the FP32 products
-  // it spans, and every cycle restarts from an FP64 residual. On the
-  // 1,024-solve batch this left the iteration counts unchanged (623.11 mean,
-  // 607 converged, C_ext within 8.9e-8 of FP64) and the helix's forward and
-  // adjoint counts identical, at half the Gram-Schmidt traffic.
-  const int passes = (fp32 && !tiling_.cgs2_fp32) ? 1 : 2;
-  for (int pass = 0; pass < passes; ++pass) {
-    launch_dots(st, nv, vi, sd, gs, vec, j, d_dot_.as<double2>());
-    // pass 0 also clears the counter k_fin_step counts into
-    launch_fin_dots(st, S, sd, gs, j, d_dot_.as<double2>(), pass, d_counter_.as<int>());
-    launch_update(st, nv, vi, sd, gs, vec, j, d_norm_.as<double>());
+  if (tiling_.mgs && !fp32) {
+    // modified Gram-Schmidt as the reference (iterative.rs:92-96): for
+    // i = 0..j, h_ij = <v_i, w> of the current w, then w -= h_ij v_i, one
+    // (dot, sum, update) triple of launches per basis vector
+    for (int i = 0; i <= j; ++i) {
+      launch_dots(st, nv, vi, sd, gs, vec, i, i, d_dot_.as<double2>());
+      launch_fin_dots(st, S, sd, gs, i, i, d_dot_.as<double2>(), 0, d_counter_.as<int>());
+      launch_update(st, nv, vi, sd, gs, vec, i, i, d_norm_.as<double>());
+    }
+  } else {
+    // FP64 steps: classical Gram-Schmidt with one reorthogonalisation pass
+    // (CGS2), orthogonal to working precision like the reference's MGS
+    // (iterative.rs:92-96), with all dots of a pass in one reduction. Steps
+    // with an FP32 matvec take one classical pass (FP64 arithmetic): the
+    // basis only has to be as good as the FP32 products it spans, and every
+    // cycle restarts from an FP64 residual. On the 1,024-solve batch this
+    // left the iteration counts unchanged (623.11 mean, 607 converged, C_ext
+    // within 8.9e-8 of FP64) and the helix's forward and adjoint counts
+    // identical, at half the Gram-Schmidt traffic.
+    const int passes = (fp32 && !tiling_.cgs2_fp32) ? 1 : 2;
+    for (int pass = 0; pass < passes; ++pass) {
+      launch_dots(st, nv, vi, sd, gs, vec, 0, j, d_dot_.as<double2>());
+      // pass 0 also clears the counter k_fin_step counts into
+      launch_fin_dots(st, S, sd, gs, 0, j, d_dot_.as<double2>(), pass, d_counter_.as<int>());
+      launch_update(st, nv, vi, sd, gs, vec, 0, j, d_norm_.as<double>());
+    }
   }
   launch_fin_step(st, S, sd, gs, j, d_norm_.as<double>(), fp32 ? 1 : 0, d_counter_.as<int>());
   launch_scale(st, nv, vi, sd, gs, 2, vec, LH_SLOT_W, -1);

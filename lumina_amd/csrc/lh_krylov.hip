@@ -98,11 +98,11 @@ __global__ __launch_bounds__(LH_VB) void k_scale(const VecItem* __restrict__ ite
   }
 }
 
-// dot_part[item][i] = sum conj(v_i) w over the item, i = 0..j.
+// dot_part[item][i] = sum conj(v_i) w over the item, i = i0..j.
 __global__ __launch_bounds__(LH_VB) void k_dots(const VecItem* __restrict__ items,
                                                const SolveDev* __restrict__ solves,
                                                const GState* __restrict__ st,
-                                               const double2* __restrict__ vec, int j,
+                                               const double2* __restrict__ vec, int i0, int j,
                                                double2* __restrict__ dot_part) {
   __shared__ double lds[2][4][LH_MMAX + 1];
   const VecItem it = items[blockIdx.x];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(LH_VB) void k_dots(const VecItem* __restrict__ item
     w[q] = ok[q] ? W[e] : make_double2(0.0, 0.0);
   }
   const int wv = threadIdx.x >> 6;
-  for (int i = 0; i <= j; ++i) {
+  for (int i = i0; i <= j; ++i) {
     const double2* Vi = V + (long long)i * n3;
     double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(LH_VB) void k_dots(const VecItem* __restrict__ item
     }
   }
   __syncthreads();
-  if (threadIdx.x <= j) {
+  if (threadIdx.x >= i0 && threadIdx.x <= j) {
     const int i = threadIdx.x;
     const double sr = (lds[0][0][i] + lds[0][1][i]) + (lds[0][2][i] + lds[0][3][i]);
     const double si = (lds[1][0][i] + lds[1][1][i]) + (lds[1][2][i] + lds[1][3][i]);
@@ -149,11 +149,11 @@ __global__ __launch_bounds__(LH_VB) void k_dots(const VecItem* __restrict__ item
   }
 }
 
-// w -= sum_{i<=j} hp_i v_i ; norm_part[item] = |w|^2 over the item.
+// w -= sum_{i0<=i<=j} hp_i v_i ; norm_part[item] = |w|^2 over the item.
 __global__ __launch_bounds__(LH_VB) void k_update(const VecItem* __restrict__ items,
                                                  const SolveDev* __restrict__ solves,
                                                  const GState* __restrict__ st,
-                                                 double2* __restrict__ vec, int j,
+                                                 double2* __restrict__ vec, int i0, int j,
                                                  double* __restrict__ norm_part) {
   __shared__ double lds[4];
   __shared__ double2 h[LH_MMAX + 1];
@@ -161,7 +161,8 @@ __global__ __launch_bounds__(LH_VB) void k_update(const VecItem* __restrict__ it
   if (!part_of(st, it.s, 0)) return;
   const SolveDev S = solves[it.s];
   const GState& g = st[it.s];
-  if (threadIdx.x <= j) h[threadIdx.x] = make_double2(g.hp_re[threadIdx.x], g.hp_im[threadIdx.x]);
+  if (threadIdx.x >= i0 && threadIdx.x <= j)
+    h[threadIdx.x] = make_double2(g.hp_re[threadIdx.x], g.hp_im[threadIdx.x]);
   __syncthreads();
   const long long n3 = 3LL * S.n;
   double2* V = vec + S.vec_off;
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(LH_VB) void k_update(const VecItem* __restrict__ it
     const long long e = e0 + threadIdx.x + q * LH_VB;
     if (e < n3) {
       double2 w = W[e];
-      for (int i = 0; i <= j; ++i) w = csub(w, cmul(V[(long long)i * n3 + e], h[i]));
+      for (int i = i0; i <= j; ++i) w = csub(w, cmul(V[(long long)i * n3 + e], h[i]));
       W[e] = w;
       acc += w.x * w.x + w.y * w.y;
     }
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(64) void k_zero(int* __restrict__ counter) {
 // Pass 0 also clears the step counter that k_fin_step of the same Arnoldi
 // step counts into (block 0, before any block leaves).
 __global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ solves,
-                                                 GState* __restrict__ st, int j,
+                                                 GState* __restrict__ st, int i0, int j,
                                                  const double2* __restrict__ dot_part,
                                                  int pass, int* __restrict__ counter) {
   const int s = blockIdx.x;
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ s
   if (!g.inner) return;
   const SolveDev S = solves[s];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = w; i <= j; i += 4) {
+  for (int i = i0 + w; i <= j; i += 4) {
     double sr = 0.0, si = 0.0;
     for (int b = lane; b < S.nvi; b += 64) {
       const double2 d = dot_part[(long long)(S.vi_off + b) * LH_NDOT + i];
@@ -511,12 +512,13 @@ void launch_scale(hipStream_t st, int nitems, const VecItem* items, const SolveD
                      src_slot, dst_slot);
 }
 void launch_dots(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
-                 const GState* gs, const double2* vec, int j, double2* dp) {
-  hipLaunchKernelGGL(k_dots, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, vec, j, dp);
+                 const GState* gs, const double2* vec, int i0, int j, double2* dp) {
+  hipLaunchKernelGGL(k_dots, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, vec, i0, j, dp);
 }
 void launch_update(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
-                   const GState* gs, double2* vec, int j, double* np) {
-  hipLaunchKernelGGL(k_update, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, vec, j, np);
+                   const GState* gs, double2* vec, int i0, int j, double* np) {
+  hipLaunchKernelGGL(k_update, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, vec, i0, j,
+                     np);
 }
 void launch_xupdate(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
                     const GState* gs, double2* vec) {
@@ -534,9 +536,10 @@ void launch_copy_slot(hipStream_t st, int nitems, const VecItem* items, const So
 void launch_zero(hipStream_t st, int* counter) {
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, counter);
 }
-void launch_fin_dots(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
+void launch_fin_dots(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int i0, int j,
                      const double2* dp, int pass, int* counter) {
-  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, j, dp, pass, counter);
+  hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, i0, j, dp, pass,
+                     counter);
 }
 void launch_fin_step(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
                      const double* np, int fp32, int* counter) {
