@@ -395,8 +395,10 @@ def run_batch(args):
             "traffic": traffic, "avg_launch_ms": far_ms / max(far_n, 1), "launches": far_n,
             "flops_per_pair": FLOPS_PER_FAR_PAIR, "targets_per_item": blk,
             "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0,
-            "events": "HIP events on the batch stream, event-record nodes of the captured "
-                      "cycle graphs, around every far-field launch of the timed steps"}
+            "events": ("HIP events on the batch stream around every far-field launch of the "
+                       "timed steps, " + ("recorded by the host-issued cycles"
+                                          if os.environ.get("LUMINA_GRAPH") == "0" else
+                                          "event-record nodes of the captured cycle graphs"))}
 
     line = {
         "metric": "Green pair-evals/s (batched mixed-precision CDA solves)",
