@@ -590,18 +590,29 @@ def run_sphere(args):
     probs = [dict(positions=pos, polarisability=I.polarisability_block("Au", lam, d, n_env),
                   k=I.wavenumber(lam, n_env), use_fcd=True, cell_size=d) for lam in wls]
     be = HipBackend(0)
-    b = Batch(be, probs)
-    for _ in range(args.warmup):
-        b.run(1e-10, args.max_iter, FP64)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run(1e-10, args.max_iter, FP64)
-    dt = (time.perf_counter() - t0) / args.steps
-    cs, infos = results_allow_failures(b)
-    b.close()
+
+    def spectrum(env):
+        for key, val in env.items():
+            os.environ[key] = val
+        b = Batch(be, probs)
+        for key in env:
+            os.environ.pop(key)
+        for _ in range(args.warmup):
+            b.run(1e-10, args.max_iter, FP64)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run(1e-10, args.max_iter, FP64)
+        dt = (time.perf_counter() - t0) / args.steps
+        cs, infos = results_allow_failures(b)
+        b.close()
+        return dt, cs, infos
+
+    # CGS2 (the default) and modified Gram-Schmidt in the reference's order
+    dt, cs, infos = spectrum({})
+    dt_m, cs_m, infos_m = spectrum({"LUMINA_ORTHO": "mgs"})
     ev = sum(i["pair_evals"] for i in infos)
     t1 = time.perf_counter()
-    errs, ev_cpu = [], 0.0
+    errs, errs_m, it_diff, ev_cpu = [], [], [], 0.0
     threads = cpu_threads()
     for q, lam in enumerate(wls):
         P = O.Problem(pos, probs[q]["polarisability"], probs[q]["k"], True, d)
@@ -609,9 +620,12 @@ def run_sphere(args):
         ev_cpu += io["matvecs"] * pairs(len(pos))
         if io["status"] == 0 and infos[q]["status"] == 0:
             errs.append(float(np.max(np.abs(cs[q] - cso) / np.abs(cso))))
+        if io["status"] == 0 and infos_m[q]["status"] == 0:
+            errs_m.append(float(np.max(np.abs(cs_m[q] - cso) / np.abs(cso[0]))))
+        it_diff.append(int(infos_m[q]["iterations"]) - int(io["iterations"]))
     t_cpu = time.perf_counter() - t1
-    log(f"[bench] sphere spectrum: GPU {dt:.3f} s, oracle {t_cpu:.1f} s, max rel err "
-        f"{max(errs) if errs else None}")
+    log(f"[bench] sphere spectrum: GPU {dt:.3f} s (CGS2), {dt_m:.3f} s (MGS), oracle {t_cpu:.1f} s, "
+        f"max rel err {max(errs) if errs else None} (CGS2), {max(errs_m) if errs_m else None} (MGS)")
     return {
         "metric": "Green pair-evals/s (31-wavelength FP64 spectrum, one batched solve)",
         "value": ev / dt, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -623,6 +637,11 @@ def run_sphere(args):
         "spectrum_nm": wls, "c_ext": cs[:, 0].tolist(), "c_abs": cs[:, 1].tolist(),
         "c_sca": cs[:, 2].tolist(),
         "max_rel_err_vs_oracle": max(errs) if errs else None,
+        "mgs": {"s": dt_m, "iterations": [i["iterations"] for i in infos_m],
+                "iterations_minus_oracle": it_diff,
+                "max_rel_err_vs_oracle": max(errs_m) if errs_m else None,
+                "note": "LUMINA_ORTHO=mgs: modified Gram-Schmidt in the reference's order "
+                        "(iterative.rs:92-96); error relative to C_ext"},
         "cpu_baseline": {"value": ev_cpu / t_cpu, "unit": "pair-evals/s", "cores": threads,
                          "kind": "port", "sample": "the same 31 solves by the oracle (FP64 "
                                                    "restatement, rows over a thread pool)"},
@@ -675,27 +694,52 @@ def run_helix(args):
                                                           rows=rows)
     t_or = time.perf_counter() - t1
     e_or = float(np.max(np.abs(gp12[rows] - gro[rows])) / np.max(np.abs(gro[rows])))
-    fd_err = 0.0
+    e_or_eps = float(np.max(np.abs(ge12[rows] - geo[rows])) / np.max(np.abs(geo[rows])))
+    # central differences on the same 32 dipoles, all perturbed problems in
+    # one FP64 batch: Re eps of each (relative step 1e-6 of |eps|), and the
+    # position of the first fd_dipoles along a seeded axis (step 1e-6 nm, FCD
+    # classification held at the unperturbed positions)
+    from lumina_amd.solver import Batch, ConvergenceFailure
+    h, hr = 1e-6 * abs(eps), 1e-6
     alphas = np.tile(a, (n, 1))
-    for i in np.random.default_rng(2).choice(n, args.fd_dipoles, replace=False):
-        h = 1e-6 * abs(eps)
-        c = []
+    probs = []
+    for i in rows:
         for sg in (1, -1):
             a2 = alphas.copy()
             a2[i] = I.polarisability_from_epsilon(eps + sg * h, lam, d, n_env)
-            ss = PairSumSession(be, pos, a2, k, True, d)
-            c.append(ss.compute_cross_sections(inc, 1e-12, args.max_iter)[0][0])
-            ss.close()
-        fd = (c[0] - c[1]) / (2 * h)
-        fd_err = max(fd_err, abs(fd - ge12[i, 0]) / np.max(np.abs(ge12[:, 0])))
+            probs.append(dict(positions=pos, polarisability=a2, k=k, use_fcd=True, cell_size=d,
+                              incident=inc))
+    nfd = min(args.fd_dipoles, len(rows))
+    axes = np.random.default_rng(4).integers(0, 3, nfd)
+    for i, ax in zip(rows[:nfd], axes):
+        for sg in (1, -1):
+            p2 = pos.copy()
+            p2[i, ax] += sg * hr
+            probs.append(dict(positions=p2, polarisability=a, k=k, use_fcd=True, cell_size=d,
+                              incident=inc, classify_positions=pos))
+    t1 = time.perf_counter()
+    b = Batch(be, probs)
+    b.run(1e-12, args.max_iter, FP64)
+    try:
+        cfd, _, _ = b.results()
+    except ConvergenceFailure as ex:
+        cfd = ex.cross_sections
+    b.close()
+    t_fd = time.perf_counter() - t1
+    ne = 2 * len(rows)
+    fd_eps = (cfd[0:ne:2, 0] - cfd[1:ne:2, 0]) / (2 * h)
+    fd_err = float(np.max(np.abs(fd_eps - ge12[rows, 0])) / np.max(np.abs(ge12[:, 0])))
+    fd_r = (cfd[ne::2, 0] - cfd[ne + 1::2, 0]) / (2 * hr)
+    fd_err_r = float(np.max(np.abs(fd_r - gp12[rows[:nfd], axes])) / np.max(np.abs(gp12)))
+    log(f"[bench] helix finite differences: {len(probs)} FP64 solves in one batch, {t_fd:.1f} s")
     g64, g32 = res["fp64"]["out"], res["mixed"]["out"]
     mixed_err = max(float(np.max(np.abs(g32[h]["gp"] - g64[h]["gp"])) / np.max(np.abs(g64[h]["gp"])))
                     for h in pols)
     cext_err = max(abs(g32[h]["cs"][0] - g64[h]["cs"][0]) / abs(g64[h]["cs"][0]) for h in pols)
     for r in res.values():
         r["session"].close()
-    log(f"[bench] helix checks: oracle rows {e_or:.2e} ({t_or:.1f} s), FD {fd_err:.2e}, "
-        f"mixed vs fp64 gradient {mixed_err:.2e}")
+    log(f"[bench] helix checks: oracle rows {e_or:.2e}/{e_or_eps:.2e} ({t_or:.1f} s), FD eps "
+        f"{fd_err:.2e}, FD r {fd_err_r:.2e}, mixed vs fp64 gradient {mixed_err:.2e}")
     m = res["mixed"]
     cpu = None
     if not args.no_cpu_baseline:
@@ -718,8 +762,12 @@ def run_helix(args):
                                 for h, o in g64.items()}},
         "mixed_iterations": {h: (o["fi"]["iterations"], o["ai"]["iterations"]) for h, o in g32.items()},
         "cext_rel_err_mixed_vs_fp64": cext_err, "grad_rel_err_mixed_vs_fp64": mixed_err,
-        "grad_check": {"oracle_rows": 32, "oracle_rel_err": e_or, "fd_dipoles": args.fd_dipoles,
-                       "fd_rel_err": fd_err, "fd_step": "1e-6 |eps|", "tol": 1e-12},
+        "grad_check": {"oracle_rows": 32, "oracle_rel_err_positions": e_or,
+                       "oracle_rel_err_eps": e_or_eps, "fd_eps_dipoles": len(rows),
+                       "fd_eps_rel_err": fd_err, "fd_eps_step": "1e-6 |eps|",
+                       "fd_position_dipoles": nfd, "fd_position_rel_err": fd_err_r,
+                       "fd_position_step_nm": hr, "fd_batch_s": t_fd, "tol": 1e-12,
+                       "bar": "1e-5 of the largest gradient component (FD), 1e-8 (oracle)"},
     }
 
 
@@ -882,7 +930,7 @@ def main():
     ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix",
                                            "fields", "sweep", "copy"],
                     default="batch")
-    ap.add_argument("--fd-dipoles", type=int, default=4)
+    ap.add_argument("--fd-dipoles", type=int, default=8)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=1000)
