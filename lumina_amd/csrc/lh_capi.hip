@@ -193,6 +193,8 @@ lh::Tiling tiling_from_env() {
   if (og && std::string(og) == "mgs") t.mgs = 1;
   const char* pp = std::getenv("LUMINA_PART64");
   if (pp && std::string(pp) == "1") t.part32 = 0;
+  const char* rt = std::getenv("LUMINA_RUN_TILES");
+  if (rt && std::atoi(rt) > 0) t.run_tiles = std::atoi(rt);
   const char* gc = std::getenv("LUMINA_GRAPH_CYCLES");
   if (gc && std::string(gc) == "all") t.graph_cycles = -1;
   else if (gc && std::atoi(gc) > 0) t.graph_cycles = std::atoi(gc);

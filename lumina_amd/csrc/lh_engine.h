@@ -56,6 +56,8 @@ struct Tiling {
   int mgs = 0;                // FP64 Arnoldi steps: modified Gram-Schmidt in the reference's
                               // order (1) instead of CGS2 (0)
   int part32 = 1;             // FP32 matvecs store their far partial slices in FP32
+  int run_tiles = 2;          // FP32 one-tensor-per-pair form: source tiles per work item
+                              // (a run of one class, near or clean, for one target tile)
   int graph_cycles = 0;       // restart cycles per graph launch: 0 = 1, 2, 4, ... up to
                               // kAutoGraphCycles; N > 0 = N; -1 = enough cycles to reach
                               // max_iter in one launch (at most kMaxGraphCycles)
@@ -200,6 +202,11 @@ class Engine {
   // free-space row items with no excluded pair (k_far_row<.., NEAR = false>)
   std::vector<FarItem> far_clean_[2];
   DevMem d_far_clean_[2];
+  // FP32 one-tensor-per-pair form in runs of source tiles: partial slots per
+  // 256-dipole tile (indexed like vit_), read by k_reduce in FP32 matvecs
+  bool runs32_ = false;
+  std::vector<int> nslot32_;
+  DevMem d_nslot32_;
   // bounding boxes of the FP32-rounded positions of every 64-dipole block of
   // each geometry: lo xyz, hi xyz
   std::vector<std::vector<double>> box64_;

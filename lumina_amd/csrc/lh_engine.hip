@@ -34,7 +34,7 @@ void launch_far_symrot64(hipStream_t, int, const FarItem*, const SolveDev*, cons
                          const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                    const int*, const int*, const double*, const double2*, double2*, int, int,
-                   const double2*, int, int, double*);
+                   const double2*, int, const int*, int, double*);
 void launch_vnorm(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
                   const double2*, int, double*);
 void launch_scale(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
@@ -405,36 +405,74 @@ void Engine::build_far_items(int p) {
   const bool rot = tiling_.block[p] == kSymRotBlock;
   const bool sym = tiling_.block[p] == kSymBlock || rot;
   const int tb = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
+  // FP32 one-tensor-per-pair form: items are runs of source tiles
+  const bool runs = p == 1 && sym && !rot;
+  std::vector<int>& nslot = nslot32_;
+  if (runs) {
+    nslot.clear();
+    for (const SolveDev& d : sd_) nslot.insert(nslot.end(), (size_t)d.nvi, d.nt);
+  }
+  const int run_max = std::max(1, tiling_.run_tiles);
   for (size_t s = 0; s < sd_.size(); ++s) {
     const SolveDev& d = sd_[s];
     if (d.fft) continue;   // G x by FFT
     const bool free_space = d.lat == 0 && !d.has_sub;
+    // tile pairs whose boxes lie beyond the exclusion radius (with a margin
+    // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
+    // decided per 256 x 256 tile pair, so that every block size sends the
+    // same pairs through the same arithmetic (the one-tensor-per-pair kernels
+    // split their items T >= S the same way)
+    auto clean = [&](int ti, int tj) {
+      const Geometry& g = geoms_[probs_[s].geom];
+      const double lim = (double)g.rskip2f * (1.0 + 1e-5);
+      const int ta = ti * LH_TILE, tz = std::min(d.n, ta + LH_TILE);
+      const int ca = tj * LH_TILE, cz = std::min(d.n, ca + LH_TILE);
+      const double gap2 =
+          box_gap2(box64_[probs_[s].geom], ta / 64, (tz + 63) / 64, ca / 64, (cz + 63) / 64);
+      return gap2 > lim && gap2 > 0.0;
+    };
+    if (runs && free_space) {
+      // target tile T: runs of up to run_max consecutive source tiles S <= T
+      // of one class; slots of T's dipoles: column sums from the tiles
+      // T' > T (slot T' - T - 1), then one row-sum slot per run
+      for (int ti = 0; ti < d.nt; ++ti) {
+        int r = 0;
+        for (int tj = 0; tj <= ti; ++r) {
+          const bool c = clean(ti, tj);
+          int tk = tj + 1;
+          while (tk <= ti && tk - tj < run_max && clean(ti, tk) == c) ++tk;
+          const FarItem f{(int)s, ti * LH_TILE, tj * LH_TILE, std::min(d.n, tk * LH_TILE),
+                          d.nt - 1 - ti + r};
+          (c ? far_clean_[p] : far_free_[p]).push_back(f);
+          tj = tk;
+        }
+        nslot[(size_t)d.vi_off + ti] = d.nt - 1 - ti + r;
+      }
+      continue;
+    }
     // lattice tables: one wave of 64 targets per item (k_ewt_shift)
     const int tbs = d.lat == 1 ? 64 : tb;
     for (int ti = 0; ti < (d.n + tbs - 1) / tbs; ++ti)
       for (int tj = 0; tj < d.nt; ++tj) {
-        FarItem f{(int)s, ti * tbs, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE)};
+        const FarItem f{(int)s, ti * tbs, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE), tj};
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
         else if (d.has_sub) far_sub_[p].push_back(f);
-        else if (!(sym && free_space) || (ti >= tj && !rot)) {
-          // items whose boxes lie beyond the exclusion radius (with a margin
-          // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
-          // decided per 256 x 256 tile pair, so that every block size sends
-          // the same pairs through the same arithmetic (the
-          // one-tensor-per-pair kernels split their items T >= S the same way)
-          const Geometry& g = geoms_[probs_[s].geom];
-          const double lim = (double)g.rskip2f * (1.0 + 1e-5);
-          const int ta = (f.t0 / LH_TILE) * LH_TILE, tz = std::min(d.n, ta + LH_TILE);
-          const double gap2 = box_gap2(box64_[probs_[s].geom], ta / 64, (tz + 63) / 64,
-                                       f.c0 / 64, (f.c1 + 63) / 64);
-          if (gap2 > lim && gap2 > 0.0) far_clean_[p].push_back(f);
-          else far_free_[p].push_back(f);
-        } else if (ti >= tj) {
+        else if (!(sym && free_space) || (ti >= tj && !rot))
+          (clean(f.t0 / LH_TILE, tj) ? far_clean_[p] : far_free_[p]).push_back(f);
+        else if (ti >= tj)
           far_free_[p].push_back(f);
-        }
       }
   }
+  if (runs) {
+    // longest runs first, so that the short ones fill the end of the launch
+    for (auto* v : {&far_free_[p], &far_clean_[p]})
+      std::stable_sort(v->begin(), v->end(), [](const FarItem& a, const FarItem& b) {
+        return a.c1 - a.c0 > b.c1 - b.c0;
+      });
+    upload(d_nslot32_, nslot, stream_);
+  }
+  if (p == 1) runs32_ = runs;
   drop_graphs();
   upload(d_far_free_[p], far_free_[p], stream_);
   upload(d_far_clean_[p], far_clean_[p], stream_);
@@ -699,7 +737,8 @@ void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
   launch_reduce(stream_, (int)vit_.size(), d_vit_.as<VecItem>(), d_solves_.as<SolveDev>(),
                 d_state_.as<GState>(), which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
                 d_tab_.as<double>(), d_ainv_.as<double2>(), d_vec_.as<double2>(), in_slot,
-                out_slot, d_part_.as<double2>(), part32(fp32), mode, d_norm_.as<double>());
+                out_slot, d_part_.as<double2>(), part32(fp32),
+                fp32 && runs32_ ? d_nslot32_.as<int>() : nullptr, mode, d_norm_.as<double>());
 }
 
 void Engine::matvec_all(int in_slot, bool fp32) {

@@ -325,12 +325,15 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 //
 // G(r_i, r_j) = G(r_j, r_i) for the point tensor, so each unordered pair is
 // evaluated once and applied both ways: y_i += G x_j and y_j += G x_i. A work
-// item is a (target tile T, source tile S) pair with T >= S, 256 threads:
-// wave w holds targets T*256 + 64 w + lane. On the diagonal tile (T == S)
+// item is a target tile T and a run of source tiles S <= T, 256 threads:
+// wave w holds targets T*256 + 64 w + lane and takes the run's source tiles
+// one after the other. On the diagonal tile (T == S)
 // wave w takes sub-blocks b >= w only, and its own sub-block for steps 1..32
 // (step 32 one way only), so each unordered pair is counted once. Forward
-// sums of T go to partial slice S, transposed sums of S to slice T (on the
-// diagonal both go to slice T). Missing sources of a partial tile carry
+// sums of T over the whole run go to slot fs of T's dipoles, transposed sums
+// of each S to slot T - S - 1 of S's dipoles (on the diagonal both go to slot
+// fs); one forward slot per run instead of one per tile pair cuts the partial
+// sums written and reduced. Missing sources of a partial tile carry
 // x = 0 and a copy of a real position: their contribution is exactly 0 (a
 // coincident pair is inside the exclusion radius and skipped).
 // The four 64-source sub-blocks of S are taken one
@@ -389,9 +392,9 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
   const float k = (float)S.k, inv_k = (float)S.inv_k;
   const float krev = (float)(S.k * 0.15915494309189535);
   const float rskip2f = S.rskip2f;
-  const bool diag = NEAR && it.t0 == it.c0;
   const double c0 = S.c0;
   const long long n3 = 3LL * S.n;
+  const int tt = it.t0 / LH_TILE;
   float hi[6], lo[6], run[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = 0.0f;
@@ -405,107 +408,112 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
     }
   };
 
-  for (int b = 0; b < 4; ++b) {
-    const int jb = it.c0 + 64 * b;
-    if (jb >= S.n) break;   // uniform over the block
-    if (tid < 128) {
-      // missing sources: x = 0 at the position of the sub-block's first source
-      const int j = jb + (tid & 63);
-      const bool sv = j < S.n;
-      const Src32& r = sp[sv ? j : jb];
-      s_a[tid] = make_float4(r.x, r.y, r.z, sv ? r.v[0] : 0.0f);
-      s_b[tid] = sv ? make_float4(r.v[1], r.v[2], r.v[3], r.v[4]) : make_float4(0, 0, 0, 0);
-      s_c[tid] = sv ? r.v[5] : 0.0f;
-    }
-    __syncthreads();
-    float ta[6] = {0, 0, 0, 0, 0, 0};
-    int home = 0;   // lane l ends holding the transposed sum of source (l + home) mod 64
+  for (int sc = it.c0; sc < it.c1; sc += LH_TILE) {
+    const bool diag = NEAR && sc == it.t0;
+    // transposed sums of source tile sc / 256 (off the diagonal)
+    const long long tslot = S.part_off + (long long)(tt - sc / LH_TILE - 1) * n3;
+    for (int b = 0; b < 4; ++b) {
+      const int jb = sc + 64 * b;
+      if (jb >= S.n) break;   // uniform over the block
+      if (tid < 128) {
+        // missing sources: x = 0 at the position of the sub-block's first source
+        const int j = jb + (tid & 63);
+        const bool sv = j < S.n;
+        const Src32& r = sp[sv ? j : jb];
+        s_a[tid] = make_float4(r.x, r.y, r.z, sv ? r.v[0] : 0.0f);
+        s_b[tid] = sv ? make_float4(r.v[1], r.v[2], r.v[3], r.v[4]) : make_float4(0, 0, 0, 0);
+        s_c[tid] = sv ? r.v[5] : 0.0f;
+      }
+      __syncthreads();
+      float ta[6] = {0, 0, 0, 0, 0, 0};
+      int home = 0;   // lane l ends holding the transposed sum of source (l + home) mod 64
 
-    // steps s0 .. s1 - 1, both directions; the source of step s + 1 is read
-    // while step s computes
-    auto steps = [&](int s0, int s1) {
-      float4 ca = s_a[lane + s0], cb = s_b[lane + s0];
-      float cc = s_c[lane + s0];
+      // steps s0 .. s1 - 1, both directions; the source of step s + 1 is read
+      // while step s computes
+      auto steps = [&](int s0, int s1) {
+        float4 ca = s_a[lane + s0], cb = s_b[lane + s0];
+        float cc = s_c[lane + s0];
 #pragma unroll 4
-      for (int s = s0; s < s1; ++s) {
-        // entry lane + s + 1 <= 127
-        const float4 na = s_a[lane + s + 1], nb = s_b[lane + s + 1];
-        const float nc = s_c[lane + s + 1];
-        const float vs[6] = {ca.w, cb.x, cb.y, cb.z, cb.w, cc};
-        const float dx = xi - ca.x, dy = yi - ca.y, dz = zi - ca.z;
-        float Ar, Ai, Br, Bi;
-        if constexpr (NEAR) {
-          const float d2f = r2f_exact(dx, dy, dz);
-          coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
-        } else {
-          coeffs32<true>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), false, k, krev, inv_k, 1.0f, Ar,
-                         Ai, Br, Bi);
-        }
-        apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
-        float t6[6];
-        apply_set<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) ta[q] = rol1_add(ta[q], t6[q]);
-        ca = na;
-        cb = nb;
-        cc = nc;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    };
-
-    if (live && !(diag && b < w)) {
-      if (diag && b == w) {
-        // steps 1..31 both ways; step 32 pairs (l, l + 32) twice over, so
-        // each lane adds its own direction only
-        steps(1, 32);
-        {
-          const int e = lane + 32;
-          const float4 A4 = s_a[e], B4 = s_b[e];
-          const float vs[6] = {A4.w, B4.x, B4.y, B4.z, B4.w, s_c[e]};
-          const float dx = xi - A4.x, dy = yi - A4.y, dz = zi - A4.z;
-          const float d2f = r2f_exact(dx, dy, dz);
+        for (int s = s0; s < s1; ++s) {
+          // entry lane + s + 1 <= 127
+          const float4 na = s_a[lane + s + 1], nb = s_b[lane + s + 1];
+          const float nc = s_c[lane + s + 1];
+          const float vs[6] = {ca.w, cb.x, cb.y, cb.z, cb.w, cc};
+          const float dx = xi - ca.x, dy = yi - ca.y, dz = zi - ca.z;
           float Ar, Ai, Br, Bi;
-          coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
+          if constexpr (NEAR) {
+            const float d2f = r2f_exact(dx, dy, dz);
+            coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
+          } else {
+            coeffs32<true>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), false, k, krev, inv_k, 1.0f, Ar,
+                           Ai, Br, Bi);
+          }
           apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+          float t6[6];
+          apply_set<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) ta[q] = rol1_add(ta[q], t6[q]);
+          ca = na;
+          cb = nb;
+          cc = nc;
+          __builtin_amdgcn_sched_barrier(0);
         }
-        fold();
-        home = 31;
-      } else {
-        steps(0, 32);
-        fold();
-        steps(32, 64);
-        fold();
-        home = 63;
+      };
+
+      if (live && !(diag && b < w)) {
+        if (diag && b == w) {
+          // steps 1..31 both ways; step 32 pairs (l, l + 32) twice over, so
+          // each lane adds its own direction only
+          steps(1, 32);
+          {
+            const int e = lane + 32;
+            const float4 A4 = s_a[e], B4 = s_b[e];
+            const float vs[6] = {A4.w, B4.x, B4.y, B4.z, B4.w, s_c[e]};
+            const float dx = xi - A4.x, dy = yi - A4.y, dz = zi - A4.z;
+            const float d2f = r2f_exact(dx, dy, dz);
+            float Ar, Ai, Br, Bi;
+            coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
+            apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+          }
+          fold();
+          home = 31;
+        } else {
+          steps(0, 32);
+          fold();
+          steps(32, 64);
+          fold();
+          home = 63;
+        }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) s_t[w][(lane + home) & 63][q] = ta[q];
-    __syncthreads();
-    // the staged sources are no longer read: the next sub-block may be staged
-    // while wave b sums this one's transposed sums (read before the next
-    // barrier, written after it)
-    if (w == b) {
-      double t[6];
+      for (int q = 0; q < 6; ++q) s_t[w][(lane + home) & 63][q] = ta[q];
+      __syncthreads();
+      // the staged sources are no longer read: the next sub-block may be staged
+      // while wave b sums this one's transposed sums (read before the next
+      // barrier, written after it)
+      if (w == b) {
+        double t[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q)
-        t[q] = ((double)s_t[0][lane][q] + (double)s_t[1][lane][q]) +
-               ((double)s_t[2][lane][q] + (double)s_t[3][lane][q]);
-      if (diag) {
+        for (int q = 0; q < 6; ++q)
+          t[q] = ((double)s_t[0][lane][q] + (double)s_t[1][lane][q]) +
+                 ((double)s_t[2][lane][q] + (double)s_t[3][lane][q]);
+        if (diag) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) tr[q] = t[q];
-      } else if (jb + lane < S.n) {
-        // transposed sums of source jb + lane go to the target tile's slice
-        const long long e =
-            S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
+          for (int q = 0; q < 6; ++q) tr[q] = t[q];
+        } else if (jb + lane < S.n) {
+          // transposed sums of source jb + lane
+          const long long e = tslot + 3LL * (jb + lane);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) put_part(part, e + c, t[2 * c] * c0, t[2 * c + 1] * c0, p32);
+          for (int c = 0; c < 3; ++c)
+            put_part(part, e + c, t[2 * c] * c0, t[2 * c + 1] * c0, p32);
+        }
       }
     }
   }
   if (tvalid) {
-    // forward sums (plus, on the diagonal, the transposed sums of the same
-    // dipole) to the source tile's slice
-    const long long e = S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
+    // forward sums of the run (plus, on the diagonal, the transposed sums of
+    // the same dipole) to slot fs
+    const long long e = S.part_off + (long long)it.fs * n3 + 3LL * i;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       put_part(part, e + c, (((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
@@ -867,7 +875,8 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
     const GState* __restrict__ st, int which, const int* __restrict__ ell_col,
     const int* __restrict__ ell_tid, const double* __restrict__ tab,
     const double2* __restrict__ ainv, double2* __restrict__ vec, int in_slot, int out_slot,
-    const double2* __restrict__ part, int p32, int mode, double* __restrict__ norm_part) {
+    const double2* __restrict__ part, int p32, const int* __restrict__ nslot, int mode,
+    double* __restrict__ norm_part) {
   __shared__ double lds[4];
   const VecItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
@@ -899,12 +908,13 @@ __global__ __launch_bounds__(LH_VB) void k_reduce(
       yi[a] = si;
     }
     // the far partial slices, one per source tile (the FFT matvec fills
-    // slice 0), each thread reading its dipole's 48 contiguous bytes per
-    // slice. Summing them in element order across the block instead (fully
-    // contiguous block-wide loads, totals passed through LDS) measured within
-    // 1% on the 1,024-solve batch and 1.7x slower on a single 38,637-dipole
-    // solve, whose 151 blocks leave CUs idle.
-    const int nslice = S.fft ? 1 : S.nt;
+    // slice 0), or with nslot the slot count of this item's tile (the FP32
+    // one-tensor-per-pair form's runs), each thread reading its dipole's
+    // contiguous bytes per slice. Summing them in element order across the
+    // block instead (fully contiguous block-wide loads, totals passed through
+    // LDS) measured within 1% on the 1,024-solve batch and 1.7x slower on a
+    // single 38,637-dipole solve, whose 151 blocks leave CUs idle.
+    const int nslice = S.fft ? 1 : (nslot ? nslot[S.vi_off + it.b] : S.nt);
     for (int ch = 0; ch < nslice; ++ch) {
       const long long e = S.part_off + (long long)ch * n3 + 3LL * i;
 #pragma unroll
@@ -1078,9 +1088,10 @@ void launch_far_sym64(hipStream_t st, bool near, int nitems, const FarItem* item
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,
                    const GState* gs, int which, const int* ell_col, const int* ell_tid,
                    const double* tab, const double2* ainv, double2* vec, int in_slot,
-                   int out_slot, const double2* part, int p32, int mode, double* norm_part) {
+                   int out_slot, const double2* part, int p32, const int* nslot, int mode,
+                   double* norm_part) {
   hipLaunchKernelGGL(k_reduce, dim3(nitems), dim3(LH_VB), 0, st, items, solves, gs, which,
-                     ell_col, ell_tid, tab, ainv, vec, in_slot, out_slot, part, p32, mode,
+                     ell_col, ell_tid, tab, ainv, vec, in_slot, out_slot, part, p32, nslot, mode,
                      norm_part);
 }
 

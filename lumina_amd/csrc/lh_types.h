@@ -106,12 +106,17 @@ struct EwRecip {
 
 // Far-field work item: targets [t0, t0 + 256) against sources [c0, c1) of
 // solve s. Row sums go to partial slice c0/256, column sums (symmetric
-// kernels, off-diagonal tiles) to slice t0/256.
+// kernels, off-diagonal tiles) to slice t0/256; except in the FP32
+// one-tensor-per-pair kernel, whose items are runs of source tiles S <= T
+// (T = t0/256): there the row sums of the whole run go to slot fs, and the
+// column sums of source tile S to slot T - S - 1 (slot layout of a tile X:
+// column sums from the tiles T > X first, then its own runs' row sums).
 struct FarItem {
   int s;
   int t0;
   int c0;
   int c1;
+  int fs;
 };
 
 // Vector work item: dipoles [t0, t0 + LH_VB) of solve s (3 complex each).
