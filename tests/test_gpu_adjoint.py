@@ -168,7 +168,7 @@ def test_autograd_function(backend):
     (2.0 * c).backward()
     s = PairSumSession(backend, pos, alphas, I.wavenumber(lam, n_env), True, d)
     cs, ge, gp, _, _ = s.adjoint_grad(IncidentField(), 1e-13, 2000, dainv)
-    assert abs(float(c) - cs[0]) < 1e-12 * abs(cs[0])
+    assert abs(c.item() - cs[0]) < 1e-12 * abs(cs[0])
     assert np.allclose(e_t.grad.numpy(), 2.0 * (ge[:, 0] + 1j * ge[:, 1]), rtol=0, atol=1e-12 * np.abs(ge).max())
     assert np.allclose(p_t.grad.numpy(), 2.0 * gp, rtol=0, atol=1e-12 * np.abs(gp).max())
 

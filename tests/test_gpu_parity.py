@@ -445,6 +445,41 @@ def test_matvec_fp32_symmetric_kernel_far_tiles(backend, nx, monkeypatch):
     assert e_row < 2e-6
 
 
+@pytest.mark.parametrize("nx", [97, 203])
+def test_fp32_symmetric_runs_of_source_tiles(backend, nx, monkeypatch):
+    """The FP32 one-tensor-per-pair kernel with 1, 2, 3, 5 and 64 source tiles
+    per work item on rods of 7 and 13 tiles (near and clean runs, a ragged last
+    tile, runs longer than a tile row): every run length against the oracle and
+    against one tile per item, and a mixed solve (which reduces through the
+    per-tile slot counts) against the oracle's FP64 GMRES."""
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(4), np.arange(4), indexing="ij"), -1)
+    pos = 1.5 * g.reshape(-1, 3).astype(float)
+    alpha = I.isotropic_block(40.0 + 10.0j)
+    k = 2 * np.pi / 480.0
+    x = rand_x(len(pos), 13)
+    P = O.Problem(pos, alpha, k, True, 1.5)
+    yo = P.matvec(x)
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", "512")
+    out = {}
+    for kt in ("1", "2", "3", "5", "64"):
+        monkeypatch.setenv("LUMINA_RUN_TILES", kt)
+        s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.5,
+                                           precision=MIXED)
+        out[kt] = s.matvec(x, MIXED)
+        if kt == "3":
+            cs, _, info = s.compute_cross_sections(IncidentField(), 1e-8, 1000)
+        s.close()
+    for kt, y in out.items():
+        e, e1 = rel(y, yo), rel(y, out["1"])
+        print(f"rod n={len(pos)} run_tiles {kt}: rel err {e:.2e}, vs one tile {e1:.2e}")
+        assert e < 2e-6
+        assert e1 < 2e-6
+    ref, _, _ = P.compute_cross_sections(1e-10, 1000)
+    print(f"mixed solve with runs of 3: {info['iterations']} iterations, C_ext {cs[0]:.6e} "
+          f"vs {ref[0]:.6e}")
+    assert np.all(np.abs(cs - ref) <= 1e-6 * np.abs(ref[0]))
+
+
 def test_stored_matrix_session(backend):
     r = np.random.default_rng(7)
     n = 300
