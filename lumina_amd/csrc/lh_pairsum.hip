@@ -374,6 +374,9 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
   __shared__ float4 s_b[128];        // v1 .. v4
   __shared__ float s_c[128];         // v5
   __shared__ float s_t[4][64][6];    // [wave][source lane][component]: transposed sums
+  // [hi, lo][component][thread]: float-float row sums, each thread's own
+  // entries, touched once per 32 steps (registers stay free for occupancy)
+  __shared__ float s_hl[2][6][256];
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
   const SolveDev S = solves[it.s];
@@ -395,17 +398,28 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
   const double c0 = S.c0;
   const long long n3 = 3LL * S.n;
   const int tt = it.t0 / LH_TILE;
-  float hi[6], lo[6], run[6];
+  float run[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) hi[q] = lo[q] = run[q] = 0.0f;
-  double tr[6] = {0, 0, 0, 0, 0, 0};   // diagonal item: transposed sum of target i
+  for (int q = 0; q < 6; ++q) {
+    run[q] = 0.0f;
+    s_hl[0][q][tid] = s_hl[1][q][tid] = 0.0f;
+  }
 
-  auto fold = [&]() {
+  // adds v[q] (and vl[q], a low part) to the row sums
+  auto add_rows = [&](const float* v, const float* vl) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      two_sum(hi[q], lo[q], run[q]);
-      run[q] = 0.0f;
+      float h = s_hl[0][q][tid], l = s_hl[1][q][tid];
+      two_sum(h, l, v[q]);
+      if (vl) l += vl[q];
+      s_hl[0][q][tid] = h;
+      s_hl[1][q][tid] = l;
     }
+  };
+  auto fold = [&]() {
+    add_rows(run, nullptr);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) run[q] = 0.0f;
   };
 
   for (int sc = it.c0; sc < it.c1; sc += LH_TILE) {
@@ -498,8 +512,15 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
           t[q] = ((double)s_t[0][lane][q] + (double)s_t[1][lane][q]) +
                  ((double)s_t[2][lane][q] + (double)s_t[3][lane][q]);
         if (diag) {
+          // transposed sums of the wave's own targets: into the row sums as
+          // float-float
+          float th[6], tl[6];
 #pragma unroll
-          for (int q = 0; q < 6; ++q) tr[q] = t[q];
+          for (int q = 0; q < 6; ++q) {
+            th[q] = (float)t[q];
+            tl[q] = (float)(t[q] - (double)th[q]);
+          }
+          add_rows(th, tl);
         } else if (jb + lane < S.n) {
           // transposed sums of source jb + lane
           const long long e = tslot + 3LL * (jb + lane);
@@ -514,10 +535,11 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
     // forward sums of the run (plus, on the diagonal, the transposed sums of
     // the same dipole) to slot fs
     const long long e = S.part_off + (long long)it.fs * n3 + 3LL * i;
+    double y[6];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      put_part(part, e + c, (((double)hi[2 * c] + (double)lo[2 * c]) + tr[2 * c]) * c0,
-               (((double)hi[2 * c + 1] + (double)lo[2 * c + 1]) + tr[2 * c + 1]) * c0, p32);
+    for (int q = 0; q < 6; ++q) y[q] = (double)s_hl[0][q][tid] + (double)s_hl[1][q][tid];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) put_part(part, e + c, y[2 * c] * c0, y[2 * c + 1] * c0, p32);
   }
 }
 
