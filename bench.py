@@ -150,11 +150,15 @@ def cpu_baseline(probs, infos64, budget_s=24.0):
         P = O.Problem(p["positions"], p["polarisability"], p["k"], True, p["cell_size"])
         x = rng.standard_normal(3 * n) + 1j * rng.standard_normal(3 * n)
         for nt, budget, store in ((threads, per_part, rate_all), (1, 0.5 * per_part, rate_one)):
+            # a warm-up pass sizes the sample to the budget; then the median of 3
             rows = min(n, 4 * nt)
             dt, done = time_rows(P, x, n, rows, nt)
             rows = int(min(n, max(rows, rows * budget / max(dt, 1e-6))))
-            dt, done = time_rows(P, x, n, rows, nt)
-            store[ip] = done * (n - 1) / dt
+            rates = []
+            for _ in range(3):
+                dt, done = time_rows(P, x, n, rows, nt)
+                rates.append(done * (n - 1) / dt)
+            store[ip] = float(np.median(rates))
     work = t_all = t_one = 0.0
     for q, p in enumerate(probs):
         i = infos64[q]
@@ -168,7 +172,8 @@ def cpu_baseline(probs, infos64, budget_s=24.0):
             "one_thread_value": work / t_one, "workload_cpu_s_all_threads": t_all,
             "workload_cpu_s_one_thread": t_one, "workload_pair_evals": work,
             "sample": f"oracle matvec rows of each of the {len(parts)} particles (middle "
-                      f"wavelength) spread over its dipoles, on {threads} threads and on 1; "
+                      f"wavelength) spread over its dipoles, on {threads} threads and on 1 "
+                      f"(median of 3 after a warm-up); "
                       f"the whole {len(probs)}-solve workload's FP64 matvecs (iterations + 2 "
                       f"cycles each) priced at those row rates (extrapolated); "
                       f"{time.perf_counter() - t_start:.1f} s of sampling"}
