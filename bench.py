@@ -414,7 +414,8 @@ def run_batch(args):
                        "finished, whose kernels exit at once")}
 
     line = {
-        "metric": "Green pair-evals/s (batched mixed-precision CDA solves)",
+        "metric": ("Green pair-evals/s (batched mixed-precision CDA solves)" if prec == MIXED
+                   else "Green pair-evals/s (batched FP64 CDA solves)"),
         "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,

@@ -784,7 +784,6 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
   const double c0 = S.c0;
   const long long n3 = 3LL * S.n;
   double run[6] = {0, 0, 0, 0, 0, 0};
-  double tr[6] = {0, 0, 0, 0, 0, 0};   // diagonal item: transposed sum of target i
 
   for (int b = 0; b < 4; ++b) {
     const int jb = it.c0 + 64 * b;
@@ -866,8 +865,9 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
       for (int q = 0; q < 6; ++q)
         t[q] = (s_t[0][lane][q] + s_t[1][lane][q]) + (s_t[2][lane][q] + s_t[3][lane][q]);
       if (diag) {
+        // transposed sums of the wave's own targets: into their row sums
 #pragma unroll
-        for (int q = 0; q < 6; ++q) tr[q] = t[q];
+        for (int q = 0; q < 6; ++q) run[q] += t[q];
       } else if (jb + lane < S.n) {
         // transposed sums of source jb + lane go to the target tile's slice
         double2* o = part + S.part_off + (long long)(it.t0 / LH_TILE) * n3 + 3LL * (jb + lane);
@@ -881,8 +881,7 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
     // dipole) to the source tile's slice
     double2* o = part + S.part_off + (long long)(it.c0 / LH_TILE) * n3 + 3LL * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      o[c] = make_double2((run[2 * c] + tr[2 * c]) * c0, (run[2 * c + 1] + tr[2 * c + 1]) * c0);
+    for (int c = 0; c < 3; ++c) o[c] = make_double2(run[2 * c] * c0, run[2 * c + 1] * c0);
   }
 }
 
