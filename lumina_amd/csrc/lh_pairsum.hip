@@ -15,7 +15,8 @@
 //    tile (no atomics, reproducible results). The free-space forms with one
 //    tensor per unordered pair (k_far_sym32 in FP32, sources staged in LDS;
 //    k_far_sym64 in FP64) also write the column sums of their sources to the
-//    slice of their target tile. With a substrate (k_far_row<R, true>,
+//    slice of their target tile (k_far_sym32: to per-tile slots, with one
+//    row-sum slot per run of source tiles). With a substrate (k_far_row<R, true>,
 //    assembly.rs:272-295) the image tensor at the mirrored source is applied
 //    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
 //    |d|^2 < 1e-20.
