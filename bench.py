@@ -646,7 +646,8 @@ def run_sphere(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
         "config": {"workload": "Au sphere r=10 nm d=2 nm (515 dipoles), water, 31 wavelengths "
-                               "400..700 nm, FCD, GMRES(30) tol 1e-10", "parallelism": "single GPU"},
+                               f"400..700 nm, FCD, GMRES(30) tol 1e-10, max_iter {args.max_iter}",
+                   "parallelism": "single GPU"},
         "solves_per_s": len(wls) / dt, "iterations": [i["iterations"] for i in infos],
         "spectrum_nm": wls, "c_ext": cs[:, 0].tolist(), "c_abs": cs[:, 1].tolist(),
         "c_sca": cs[:, 2].tolist(),
