@@ -485,6 +485,8 @@ int lh_pairsum_set_wavelength(lh_pairsum* s, double k, const lh_c64* alpha, int 
 int lh_pairsum_matvec(lh_pairsum* s, const lh_c64* x, lh_c64* y, int precision) {
   return guarded([&]() {
     if (!s || !x || !y) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (precision != LH_PREC_FP64 && precision != LH_PREC_MIXED)
+      return fail(LH_INVALID_ARGUMENT, "unknown precision");
     LH_HIP(hipSetDevice(s->b->device));
     s->eng->upload_slot(0, 0, x);
     s->eng->matvec_all(0, precision == LH_PREC_MIXED);
@@ -543,6 +545,7 @@ int lh_pairsum_compute_cross_sections(lh_pairsum* s, const lh_incident* inc, dou
                                       lh_solve_info* info) {
   return guarded([&]() {
     if (!s || !inc || !out) return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
     if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
     LH_HIP(hipSetDevice(s->b->device));
     s->eng->fill_incident(0, *inc);
@@ -646,6 +649,7 @@ int lh_pairsum_adjoint_grad(lh_pairsum* s, const lh_incident* inc, double tol, s
   return guarded([&]() {
     if (!s || !inc || !dainv_deps || !cross_sections || !grad_eps)
       return fail(LH_INVALID_ARGUMENT, "null argument");
+    if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
     if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
     // with a lattice, G_per(-r) != G_per(r) for pairs at equal height
     // (sgn 0 = +1 in the spectral sum), so A is not symmetric
@@ -848,6 +852,7 @@ int lh_sweep_solve(lh_backend* b, const lh_sweep_spec* spec, const lh_material* 
     if (!b || !spec || !materials || n_materials == 0 || !wavelengths_nm || n_wl == 0 || !inc ||
         !cross_sections)
       return fail(LH_INVALID_ARGUMENT, "bad argument");
+    if (!(tol >= 0.0)) return fail(LH_INVALID_ARGUMENT, "tolerance must be >= 0");
     if (!(inc->amplitude != 0.0)) return fail(LH_INVALID_ARGUMENT, "amplitude must be non-zero");
     if (!(spec->n_env > 0.0) || !std::isfinite(spec->n_env))
       return fail(LH_INVALID_ARGUMENT, "n_env must be positive and finite");

@@ -72,6 +72,21 @@ def test_pairsum_call_errors(backend):
     assert st == _lib.LH_INVALID_ARGUMENT
     st, _ = status_of(s.set_wavelength, -2.0, alpha)
     assert st == _lib.LH_INVALID_ARGUMENT
+    # an unknown precision code, a negative or NaN tolerance
+    y = np.zeros(n3, complex)
+    st = lib.lh_pairsum_matvec(s._h, rhs.ctypes.data_as(C.c_void_p),
+                               y.ctypes.data_as(C.c_void_p), 7)
+    assert st == _lib.LH_INVALID_ARGUMENT
+    inc = IncidentField()._c()
+    out = np.zeros(3)
+    for bad_tol in (-1.0, float("nan")):
+        st = lib.lh_pairsum_compute_cross_sections(s._h, C.byref(inc), bad_tol, 100,
+                                                   out.ctypes.data_as(C.POINTER(C.c_double)), None,
+                                                   None)
+        assert st == _lib.LH_INVALID_ARGUMENT
+    # the session is still usable
+    cs, _, info = s.compute_cross_sections(IncidentField(), 1e-10, 1000)
+    assert info["status"] == 0 and cs[0] > 0.0
     s.close()
 
 
