@@ -7,13 +7,15 @@ each at 16 wavelengths 400..800 nm: 1024 GMRES(30) solves of uneven size,
 solved together by the batched device engine in mixed precision (FP32
 far-field pair sum, FP64 Krylov/Givens/residuals/near field).
 
-A step is one batched solve of all 1024 problems (inputs resident in HBM).
-Printed: one JSON line with Green pair-evals/s (the value), solves/s, the
-plain FP64 one-solve-per-launch rate on a subset of the same problems, the
-C_ext error of the mixed results against FP64 solves of the same problems
-(the problems at 4 of the 16 wavelengths, one FP64 batch, same tolerance),
-the far-field kernel roofline from HIP events, and the CPU baseline (the
-oracle's FP64 restatement of the reference CPU path on this host's cores).
+A step is one batched solve of all 1024 problems (inputs resident in HBM),
+issued as one hipGraph launch that holds every restart cycle up to max_iter.
+Printed: one JSON line with Green pair-evals/s (the value), solves/s and
+converged solves/s, the same batch with adaptive graph launches, the plain
+FP64 one-solve-per-session rate on a subset of the same problems, the C_ext
+error of the mixed results against an FP64 batch of all 1024 problems (same
+tolerance), the far-field kernel roofline from HIP events, and the CPU
+baseline (the oracle's FP64 restatement of the reference CPU path on this
+host's cores).
 
 Other workloads: --workload cuboid (80x40x40 nm Au cuboid, 1.5 nm spacing,
 38,637 dipoles, 520 nm, one solve FP64 and mixed); --workload metasurface
@@ -255,6 +257,9 @@ def run_batch(args):
     # far-field HIP events, recorded as nodes of the captured cycle graphs:
     # the timed path is the graph path that ships
     batch.set_timing(True)
+    # the whole batch solve as one graph launch (every restart cycle up to
+    # max_iter captured once) and one host read
+    batch.set_graph_cycles(-1)
 
     prec = MIXED if args.precision == "mixed" else FP64
     for w in range(args.warmup):
@@ -272,12 +277,16 @@ def run_batch(args):
         times.append(dt)
         cs_mixed, infos = results_allow_failures(batch)
         evals += sum(i["pair_evals"] for i in infos)
-        rounds += max((i["matvecs_fp32"] if prec == MIXED else i["matvecs"] - i["matvecs_fp32"])
-                      for i in infos)
+        # far-field launches with work: each solve group launches its own,
+        # one per lock-step round up to the group's longest solve
+        for g in sorted({i["solve_group"] for i in infos}):
+            rounds += max((i["matvecs_fp32"] if prec == MIXED else i["matvecs"] - i["matvecs_fp32"])
+                          for i in infos if i["solve_group"] == g)
         kt = batch.kernel_times()
         for key in kt_tot:
             kt_tot[key] += kt[key]
         trips.append((kt["host_round_trips"], kt["cycles_issued"]))
+        groups = kt["solve_groups"]
         blocks = (kt["far64_block"], kt["far32_block"], kt["tune_ms"])
         log(f"[bench] step {s_}: {dt:.2f} s, iters mean "
             f"{np.mean([i['iterations'] for i in infos]):.1f}, max "
@@ -287,32 +296,28 @@ def run_batch(args):
     value = evals / total
     iters = [i["iterations"] for i in infos]
     conv = np.array([i["status"] == 0 for i in infos])
-    batch.close()
 
-    # the whole solve as one graph launch (every cycle up to max_iter in one
-    # captured graph), one warm-up and one timed run
-    one_launch = None
-    if not args.no_one_launch:
-        os.environ["LUMINA_GRAPH_CYCLES"] = "all"
-        if blocks[0] == 512 and blocks[1] == 512:
-            os.environ["LUMINA_FAR_BLOCK"] = "sym"   # the forms the timed batch chose
-        b1 = Batch(be, probs)
-        os.environ.pop("LUMINA_GRAPH_CYCLES")
-        os.environ.pop("LUMINA_FAR_BLOCK", None)
-        b1.run(args.tol, args.max_iter, prec)
+    # the same batch with adaptive graph launches (1, 2, 4, then 8 restart
+    # cycles per launch, one host read after each): one warm-up run that
+    # captures those graphs, one timed run
+    adaptive = None
+    if not args.no_adaptive:
+        batch.set_timing(False)
+        batch.set_graph_cycles(0)
+        batch.run(args.tol, args.max_iter, prec)
         t1 = time.perf_counter()
-        b1.run(args.tol, args.max_iter, prec)
+        batch.run(args.tol, args.max_iter, prec)
         dt1 = time.perf_counter() - t1
-        cs1, inf1 = results_allow_failures(b1)
-        kt1 = b1.kernel_times()
-        b1.close()
-        one_launch = {"pair_evals_per_s": sum(i["pair_evals"] for i in inf1) / dt1,
-                      "s": dt1, "host_round_trips": kt1["host_round_trips"],
-                      "cycles_issued": kt1["cycles_issued"],
-                      "identical_to_timed_path": bool(np.array_equal(cs1, cs_mixed))}
-        log(f"[bench] one graph launch per solve: {dt1:.2f} s, "
-            f"{kt1['host_round_trips']} host round trip(s), same results: "
-            f"{one_launch['identical_to_timed_path']}")
+        cs1, inf1 = results_allow_failures(batch)
+        kt1 = batch.kernel_times()
+        adaptive = {"pair_evals_per_s": sum(i["pair_evals"] for i in inf1) / dt1,
+                    "s": dt1, "host_round_trips": kt1["host_round_trips"],
+                    "cycles_issued": kt1["cycles_issued"],
+                    "identical_to_timed_path": bool(np.array_equal(cs1, cs_mixed))}
+        log(f"[bench] adaptive graph launches: {dt1:.2f} s, "
+            f"{kt1['host_round_trips']} host round trips, same results: "
+            f"{adaptive['identical_to_timed_path']}")
+    batch.close()
 
     # FP64 answer of every solve, one FP64 batch at the same tolerance (the
     # FP64 far-field form the mixed batch's tuner chose for its residuals)
@@ -434,11 +439,12 @@ def run_batch(args):
         "max_iter": args.max_iter,
         "host_round_trips_per_step": trips[-1][0] if trips else None,
         "cycles_issued_per_step": trips[-1][1] if trips else None,
-        "graph_cycles_per_launch": "1, 2, 4, then 8 (LUMINA_GRAPH_CYCLES unset)",
+        "graph_cycles_per_launch": "every cycle up to max_iter (lh_batch_set_graph_cycles -1)",
+        "solve_groups": groups,
         "cext_rel_err_max_vs_fp64": err,
         "cext_err_covers": (f"{fp64_batch['cext_err_solves']} of {S} solves (those that "
                             f"converge in both precisions)") if fp64_batch else None,
-        "fp64_batched": fp64_batch, "fp64_plain": plain, "one_graph_launch": one_launch,
+        "fp64_batched": fp64_batch, "fp64_plain": plain, "adaptive_graph_launches": adaptive,
         "speedup_vs_fp64_plain_pair_evals": (value / plain["pair_evals_per_s"]) if plain else None,
         "roofline": roof,
     }
@@ -952,7 +958,8 @@ def main():
     ap.add_argument("--particles", type=int, default=64)
     ap.add_argument("--wavelengths", type=int, default=16)
     ap.add_argument("--plain-wavelengths", type=int, default=4)
-    ap.add_argument("--no-one-launch", action="store_true")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive-graph-launch leg of the batch workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-plain", action="store_true")
     ap.add_argument("--no-error", action="store_true")

@@ -142,7 +142,8 @@ typedef struct {
  * matvecs_fp32 = how many of those used the FP32 far field;
  * residual = final ||rhs - A x|| / ||rhs|| (FP64);
  * pair_evals = Green's pair evaluations performed (N(N-1) per matvec, plus
- * N^2 image pairs with a substrate). */
+ * N^2 image pairs with a substrate); solve_group = the solve group of a
+ * batch whose stream issued this solve's kernels (0 outside batches). */
 typedef struct {
   uint64_t iterations;
   uint64_t cycles;
@@ -151,6 +152,7 @@ typedef struct {
   double residual;
   double pair_evals;
   int status;                     /* LH_OK or LH_CONVERGENCE_FAILURE */
+  int32_t solve_group;
 } lh_solve_info;
 
 /* Builds the device state for the closure of cda/mod.rs:329-345 / 375-391
@@ -316,6 +318,15 @@ typedef struct lh_batch lh_batch;
 int lh_batch_create(lh_backend* backend, const lh_batch_problem* problems, size_t count,
                     lh_batch** out);
 int lh_batch_run(lh_batch* batch, double tol, size_t max_iter, int precision);
+/* No reference counterpart: how many restart cycles one graph launch of
+ * lh_batch_run replays. cycles = 0 (the default): 1, 2, 4, then 8 per
+ * launch, one host read of the running-solve counter after each; cycles > 0:
+ * that many (at most 64); cycles = -1: every cycle up to max_iter in one
+ * launch, so that the whole batch is solved by one hipGraphLaunch and one
+ * host read (solves that finish early skip the rest of the graph, at the
+ * cost of one status read per work item). Results are identical for every
+ * setting. LH_INVALID_ARGUMENT for cycles < -1. */
+int lh_batch_set_graph_cycles(lh_batch* batch, int cycles);
 int lh_batch_results(lh_batch* batch, double* cross_sections, lh_c64* const* moments,
                      lh_solve_info* infos);
 void lh_batch_destroy(lh_batch* batch);
@@ -328,7 +339,11 @@ void lh_batch_destroy(lh_batch* batch);
  * N^2 image pairs, per participating solve). host_round_trips = graph
  * launches (or host-issued cycles) of the last lh_batch_run, each followed
  * by one read of the running-solve counter; cycles_issued = restart cycles
- * they issued (a graph launch replays several). */
+ * they issued (a graph launch replays several). solve_groups = 1, or 2 when
+ * the batch's solves were split into two groups of about equal pair work
+ * whose restart cycles run on two streams, the far fields of the groups
+ * taking turns (each group's reduction and Gram-Schmidt run beside the other
+ * group's far field); the far-field launches are then counted per group. */
 typedef struct {
   double far64_ms, far32_ms;
   uint64_t far64_launches, far32_launches;
@@ -338,7 +353,7 @@ typedef struct {
   int32_t far32_block;
   double tune_ms;         /* time spent choosing them (0 when fixed) */
   int32_t host_round_trips;
-  int32_t reserved;
+  int32_t solve_groups;
   uint64_t cycles_issued;
 } lh_kernel_times;
 int lh_batch_set_timing(lh_batch* batch, int enable);

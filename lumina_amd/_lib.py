@@ -36,6 +36,7 @@ EXPORTS = [
     "lh_pairsum_field_at_points_noalloc", "lh_pairsum_far_field",
     "lh_pairsum_destroy", "lh_batch_solve", "lh_batch_create", "lh_batch_run",
     "lh_batch_results", "lh_batch_destroy", "lh_batch_set_timing", "lh_batch_kernel_times",
+    "lh_batch_set_graph_cycles",
     "lh_sweep_solve",
 ]
 
@@ -80,7 +81,7 @@ class Incident(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("cycles", C.c_uint64), ("matvecs", C.c_uint64),
                 ("matvecs_fp32", C.c_uint64), ("residual", C.c_double),
-                ("pair_evals", C.c_double), ("status", C.c_int)]
+                ("pair_evals", C.c_double), ("status", C.c_int), ("solve_group", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -110,7 +111,7 @@ class KernelTimes(C.Structure):
                 ("far64_pairs", C.c_double), ("far32_pairs", C.c_double),
                 ("run_ms", C.c_double), ("far64_block", C.c_int32),
                 ("far32_block", C.c_int32), ("tune_ms", C.c_double),
-                ("host_round_trips", C.c_int32), ("reserved", C.c_int32),
+                ("host_round_trips", C.c_int32), ("solve_groups", C.c_int32),
                 ("cycles_issued", C.c_uint64)]
 
     def as_dict(self):
@@ -180,6 +181,7 @@ def load():
     lib.lh_batch_destroy.argtypes = [P]
     lib.lh_batch_destroy.restype = None
     lib.lh_batch_set_timing.argtypes = [P, C.c_int]
+    lib.lh_batch_set_graph_cycles.argtypes = [P, C.c_int]
     lib.lh_batch_kernel_times.argtypes = [P, C.POINTER(KernelTimes)]
     lib.lh_sweep_solve.argtypes = [P, C.POINTER(SweepSpec), C.POINTER(Material), C.c_size_t, D,
                                    C.c_size_t, C.POINTER(Incident), C.c_double, C.c_size_t, D,

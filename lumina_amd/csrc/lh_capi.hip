@@ -177,7 +177,10 @@ lh::Problem make_problem(const lh_pairsum_spec* s, int geom) {
 // LUMINA_EWALD=pairs evaluates the Ewald tensor per pair even where the
 // dipoles sit on a grid (default: grid tables); LUMINA_FFT=0 keeps the pair
 // sum for full-grid solves, LUMINA_FFT=force takes the FFT matvec for any
-// full grid of two or more dipoles (default: from 4,096 dipoles).
+// full grid of two or more dipoles (default: from 4,096 dipoles);
+// LUMINA_GROUPS=1|2 fixes the number of solve groups whose cycles run on
+// separate streams (default: two for batches of 1e8 pair evaluations per
+// matvec or more).
 lh::Tiling tiling_from_env() {
   lh::Tiling t;
   const char* sc = std::getenv("LUMINA_FP32_SINCOS");
@@ -198,6 +201,8 @@ lh::Tiling tiling_from_env() {
   const char* gc = std::getenv("LUMINA_GRAPH_CYCLES");
   if (gc && std::string(gc) == "all") t.graph_cycles = -1;
   else if (gc && std::atoi(gc) > 0) t.graph_cycles = std::atoi(gc);
+  const char* gp = std::getenv("LUMINA_GROUPS");
+  if (gp && (std::atoi(gp) == 1 || std::atoi(gp) == 2)) t.groups = std::atoi(gp);
   const char* cg = std::getenv("LUMINA_CGS2");
   if (cg && std::string(cg) == "1") t.cgs2_fp32 = 1;
   const char* bl = std::getenv("LUMINA_FAR_BLOCK");
@@ -226,6 +231,7 @@ void fill_info(const lh::SolveResult& r, lh_solve_info* info) {
   info->residual = r.residual;
   info->pair_evals = r.pair_evals;
   info->status = r.status;
+  info->solve_group = r.group;
 }
 
 }  // namespace
@@ -816,6 +822,15 @@ int lh_batch_set_timing(lh_batch* bt, int enable) {
   });
 }
 
+int lh_batch_set_graph_cycles(lh_batch* bt, int cycles) {
+  return guarded([&]() {
+    if (!bt) return fail(LH_INVALID_ARGUMENT, "null batch");
+    if (cycles < -1) return fail(LH_INVALID_ARGUMENT, "cycles must be -1, 0 or positive");
+    bt->eng->set_graph_cycles(cycles);
+    return (int)LH_OK;
+  });
+}
+
 int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
   return guarded([&]() {
     if (!bt || !out) return fail(LH_INVALID_ARGUMENT, "bad argument");
@@ -838,6 +853,7 @@ int lh_batch_kernel_times(lh_batch* bt, lh_kernel_times* out) {
     out->far32_block = bt->eng->block(1);
     out->tune_ms = bt->eng->tune_ms;
     out->host_round_trips = bt->eng->last_launches;
+    out->solve_groups = bt->eng->groups();
     out->cycles_issued = (uint64_t)bt->eng->last_cycles_issued;
     return (int)LH_OK;
   });

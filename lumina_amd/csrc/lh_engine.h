@@ -61,7 +61,10 @@ struct Tiling {
   int graph_cycles = 0;       // restart cycles per graph launch: 0 = 1, 2, 4, ... up to
                               // kAutoGraphCycles; N > 0 = N; -1 = enough cycles to reach
                               // max_iter in one launch (at most kMaxGraphCycles)
+  int groups = 0;             // solve groups: 1, 2, or 0 = two when the batch has at least
+                              // two solves and kGroupMinPairs pair evaluations per matvec
 };
+constexpr double kGroupMinPairs = 1e8;
 constexpr int kAutoGraphCycles = 8;
 constexpr int kMaxGraphCycles = 64;
 
@@ -82,6 +85,7 @@ struct SweepDevArgs {
 
 struct SolveResult {
   int status = 0;
+  int group = 0;
   int iterations = 0, cycles = 0, matvecs = 0, matvecs32 = 0;
   double residual = 0.0;
   double pair_evals = 0.0;
@@ -157,6 +161,9 @@ class Engine {
   int last_launches = 0;
   long long last_cycles_issued = 0;
   int block(int fp32) const { return tiling_.block[fp32]; }
+  int groups() const { return ng_; }
+  // restart cycles per graph launch (Tiling::graph_cycles)
+  void set_graph_cycles(int c) { tiling_.graph_cycles = c; }
   int fft_solves() const { return (int)fft_.size(); }
 
  private:
@@ -170,14 +177,48 @@ class Engine {
   long long far_n_[2] = {0, 0};
   bool timing_ = false;
   void clear_events();
-  void record(hipEvent_t e);
+  void record(hipEvent_t e, hipStream_t st);
   // adds the elapsed times of evs to the far-field sums (destroy: and frees them)
   void harvest(std::vector<EvPair>& evs, bool destroy);
-  void far_field(int p, int which, int in_slot);
+  // g = -1: every solve on stream_; g = 0, 1: the solves of group g on its
+  // stream
+  void far_field(int p, int which, int in_slot, int g = -1);
   int part32(bool fp32) const { return (fp32 && tiling_.part32) ? 1 : 0; }
-  void matvec(int which, int in_slot, int out_slot, int mode, bool fp32);
-  void arnoldi_step(int j, bool fp32);
+  void matvec(int which, int in_slot, int out_slot, int mode, bool fp32, int g = -1);
+  void reduce(int which, int in_slot, int out_slot, int mode, bool fp32, int g);
+  // an Arnoldi step after its far field: reduction, Gram-Schmidt, Givens,
+  // normalisation
+  void step_rest(int j, bool fp32, int g);
+  void arnoldi_step(int j, bool fp32, int g);
   void enqueue_cycle(bool mixed, bool early_exit);
+  void enqueue_cycle_groups(bool mixed, bool early_exit);
+
+  // Solve groups. With two, the solves are split into two groups of about
+  // equal pair work, whose restart cycles are issued on two streams
+  // (stream_ and gst_[1]); the far fields of the groups take turns (group 0
+  // step j, group 1 step j, group 0 step j + 1, ...), so that one group's
+  // reduction, Gram-Schmidt and normalisation run beside the other group's
+  // far field. Every solve performs exactly the arithmetic of the one-group
+  // engine, in the same order: results are identical.
+  int ng_ = 1;
+  hipStream_t gst_[2] = {nullptr, nullptr};
+  std::vector<int> order_;          // solves in group order
+  std::vector<int> group_of_;
+  int glo_[3] = {0, 0, 0};          // group g: order_[glo_[g] .. glo_[g + 1])
+  int vlo_[3] = {0, 0, 0};          // group g's vector items: vit_[vlo_[g] .. vlo_[g + 1])
+  int flo_[2][5][3] = {};           // far lists [precision][free, clean, sub, ewt, ewp]:
+                                    // group g's items [flo_[p][l][g], flo_[p][l][g + 1])
+  DevMem d_order_;
+  hipEvent_t gev_[3] = {nullptr, nullptr, nullptr};   // far field of group 0 / 1 done; join
+  hipStream_t gstream(int g) const { return g <= 0 ? stream_ : gst_[1]; }
+  int vlo(int g) const { return g < 0 ? 0 : vlo_[g]; }
+  int vcount(int g) const { return g < 0 ? (int)vit_.size() : vlo_[g + 1] - vlo_[g]; }
+  const int* slist(int g) const { return g < 0 ? nullptr : d_order_.as<int>() + glo_[g]; }
+  int scount(int g) const { return g < 0 ? (int)sd_.size() : glo_[g + 1] - glo_[g]; }
+  int* counter(int g) const { return d_counter_.as<int>() + (g < 0 ? 0 : g); }
+  // stream b waits for what stream a has issued so far (event e)
+  void wait_for(hipStream_t b, hipStream_t a, hipEvent_t e);
+  void join_groups();
   // K restart cycles captured once as a HIP graph and replayed, keyed by
   // (K, timing events, precision)
   struct CycleGraph {
@@ -241,7 +282,7 @@ class Engine {
   std::vector<std::unique_ptr<FftSolve>> fft_;
   void setup_fft();                       // plans and index maps
   void fft_kernels();                     // spectra of the tensor for the current k
-  void fft_matvec(int in_slot, bool fp32);
+  void fft_matvec(int in_slot, bool fp32, int g, hipStream_t st);
   bool has_pair_work() const;             // any solve on the pair-sum kernels
   long long n_tab_total_ = 0;
   int n_ewitems_ = 0;

@@ -49,10 +49,12 @@ void launch_cross(hipStream_t, int, const VecItem*, const SolveDev*, const doubl
 void launch_copy_slot(hipStream_t, int, const VecItem*, const SolveDev*, double2*, int, int);
 void launch_zero(hipStream_t, int*);
 void launch_fin_dots(hipStream_t, int, const SolveDev*, GState*, int, int, const double2*, int,
-                     int*);
-void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const double*, int, int*);
-void launch_backsub(hipStream_t, int, GState*);
-void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*);
+                     int*, const int*);
+void launch_fin_step(hipStream_t, int, const SolveDev*, GState*, int, const double*, int, int*,
+                     const int*);
+void launch_backsub(hipStream_t, int, GState*, const int*);
+void launch_check(hipStream_t, int, const SolveDev*, GState*, const double*, int, int, int*,
+                  const int*);
 
 void launch_ewald_table(hipStream_t, int, const SolveDev*, const int*, const long long*,
                         const double*, double*, float*);
@@ -78,9 +80,10 @@ void launch_fft_scatter64(hipStream_t, long long, const int*, const double2*, do
 void launch_fft_scatter32(hipStream_t, long long, const int*, const double2*, float2*);
 void launch_fft_mul64(hipStream_t, long long, const double2*, double2*);
 void launch_fft_mul32(hipStream_t, long long, const float2*, float2*);
-void launch_fft_gather64(hipStream_t, int, long long, const int*, const double2*, double, double2*);
+void launch_fft_gather64(hipStream_t, int, long long, const int*, const double2*, double, double2*,
+                         long long);
 void launch_fft_gather32(hipStream_t, int, long long, const int*, const float2*, double, double2*,
-                         int);
+                         long long, int);
 void launch_sweep_setup(hipStream_t, const SweepDevArgs&, const SolveDev*, int, double2*);
 void launch_sweep_incident(hipStream_t, const SolveDev*, int, int, const double*, const double*,
                            const double*, const double[3], double, const double*, double2*);
@@ -144,9 +147,13 @@ static void upload(DevMem& m, const std::vector<T>& v, hipStream_t s) {
 // device memory. The owner destroys the stream last.
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
+  if (gst_[1]) (void)hipStreamSynchronize(gst_[1]);
   clear_events();
   drop_graphs();
   fft_.clear();
+  for (auto& e : gev_)
+    if (e) (void)hipEventDestroy(e);
+  if (gst_[1]) (void)hipStreamDestroy(gst_[1]);
 }
 
 namespace {
@@ -224,6 +231,45 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     }
   }
 
+  // solve groups: two of about equal pair work (largest solves first, each
+  // to the lighter group), solves of a group in index order
+  {
+    std::vector<double> cost(S);
+    double tot = 0.0;
+    for (int s = 0; s < S; ++s) {
+      const double n = (double)geoms_[probs_[s].geom].n;
+      cost[s] = n * n * (probs_[s].has_sub ? 2.0 : 1.0);
+      tot += cost[s];
+    }
+    const int want = tiling_.groups > 0 ? tiling_.groups : (tot >= kGroupMinPairs ? 2 : 1);
+    ng_ = (want >= 2 && S >= 2) ? 2 : 1;
+    group_of_.assign(S, 0);
+    if (ng_ == 2) {
+      std::vector<int> by(S);
+      for (int s = 0; s < S; ++s) by[s] = s;
+      std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+      double load[2] = {0.0, 0.0};
+      for (int s : by) {
+        const int g = load[1] < load[0] ? 1 : 0;
+        group_of_[s] = g;
+        load[g] += cost[s];
+      }
+    }
+    order_.clear();
+    for (int g = 0; g < ng_; ++g) {
+      glo_[g] = (int)order_.size();
+      for (int s = 0; s < S; ++s)
+        if (group_of_[s] == g) order_.push_back(s);
+    }
+    glo_[ng_] = (int)order_.size();
+    if (ng_ == 1) glo_[2] = glo_[1];
+    if (ng_ == 2 && !gst_[1]) {
+      LH_HIP(hipStreamCreateWithFlags(&gst_[1], hipStreamNonBlocking));
+      for (auto& e : gev_) LH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    gst_[0] = stream_;
+  }
+
   sd_.assign(S, SolveDev{});
   vit_.clear();
   long long vec_off = 0, src_off = 0, part_off = 0, tab_off = 0, ainv_off = 0;
@@ -255,7 +301,6 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     d.vec_off = vec_off;
     d.src_off = src_off;
     d.part_off = part_off;
-    d.vi_off = vi_off;
     d.nvi = (g.n + LH_VB - 1) / LH_VB;
     d.max_iter = 0;
     d.rskip2f = g.rskip2f;
@@ -277,7 +322,6 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
       std::vector<int> offs;
       d.fft = fcd_offsets(g, gidx, offs) ? 2 : 1;
     }
-    for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
     for (int u = 0; u < g.ntab; u += 256) {
       tis.push_back(s);
       tiu.push_back(u);
@@ -324,8 +368,20 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
     vec_off += (long long)LH_NSLOT * 3 * g.n;
     src_off += g.n;
     part_off += (long long)d.nt * 3 * g.n;
-    vi_off += d.nvi;
   }
+  // vector items in group order: each group's items are one range of vit_
+  for (int g = 0; g < ng_; ++g) {
+    vlo_[g] = vi_off;
+    for (int oi = glo_[g]; oi < glo_[g + 1]; ++oi) {
+      const int s = order_[oi];
+      SolveDev& d = sd_[s];
+      d.vi_off = vi_off;
+      for (int b = 0; b < d.nvi; ++b) vit_.push_back(VecItem{s, b * LH_VB, b, 0});
+      vi_off += d.nvi;
+    }
+  }
+  vlo_[ng_] = vi_off;
+  if (ng_ == 1) vlo_[2] = vlo_[1];
   n_titems_ = (int)tis.size();
   n_ewitems_ = (int)ewis.size();
   n_tab_total_ = tab_off;
@@ -338,6 +394,7 @@ void Engine::build(std::vector<Geometry>& geoms, std::vector<Problem>& probs, co
   upload(d_ell_tid_, etid, st);
   upload(d_disp_, disp, st);
   upload(d_ainv_, ainv_all, st);
+  upload(d_order_, order_, st);
   for (int q = 0; q < 2; ++q) build_far_items(q);
   upload(d_vit_, vit_, st);
   upload(d_tis_, tis, st);
@@ -409,11 +466,19 @@ void Engine::build_far_items(int p) {
   const bool runs = p == 1 && sym && !rot;
   std::vector<int>& nslot = nslot32_;
   if (runs) {
-    nslot.clear();
-    for (const SolveDev& d : sd_) nslot.insert(nslot.end(), (size_t)d.nvi, d.nt);
+    nslot.assign(vit_.size(), 0);
+    for (const SolveDev& d : sd_)
+      std::fill(nslot.begin() + d.vi_off, nslot.begin() + d.vi_off + d.nvi, d.nt);
   }
   const int run_max = std::max(1, tiling_.run_tiles);
-  for (size_t s = 0; s < sd_.size(); ++s) {
+  std::vector<FarItem>* lists[5] = {&far_free_[p], &far_clean_[p], &far_sub_[p], &far_ewt_[p],
+                                    &far_ewp_[p]};
+  // solves in group order: each group's items are one range of every list
+  for (int oi = 0; oi < (int)order_.size(); ++oi) {
+    for (int g = 0; g < ng_; ++g)
+      if (oi == glo_[g])
+        for (int l = 0; l < 5; ++l) flo_[p][l][g] = (int)lists[l]->size();
+    const size_t s = (size_t)order_[oi];
     const SolveDev& d = sd_[s];
     if (d.fft) continue;   // G x by FFT
     const bool free_space = d.lat == 0 && !d.has_sub;
@@ -464,12 +529,17 @@ void Engine::build_far_items(int p) {
           far_free_[p].push_back(f);
       }
   }
+  for (int l = 0; l < 5; ++l)
+    for (int g = ng_; g < 3; ++g) flo_[p][l][g] = (int)lists[l]->size();
   if (runs) {
-    // longest runs first, so that the short ones fill the end of the launch
-    for (auto* v : {&far_free_[p], &far_clean_[p]})
-      std::stable_sort(v->begin(), v->end(), [](const FarItem& a, const FarItem& b) {
-        return a.c1 - a.c0 > b.c1 - b.c0;
-      });
+    // longest runs first within each group, so that the short ones fill the
+    // end of its launch
+    for (int l = 0; l < 2; ++l)
+      for (int g = 0; g < ng_; ++g)
+        std::stable_sort(lists[l]->begin() + flo_[p][l][g], lists[l]->begin() + flo_[p][l][g + 1],
+                         [](const FarItem& a, const FarItem& b) {
+                           return a.c1 - a.c0 > b.c1 - b.c0;
+                         });
     upload(d_nslot32_, nslot, stream_);
   }
   if (p == 1) runs32_ = runs;
@@ -652,28 +722,39 @@ double Engine::pairs_per_matvec(int s) const {
 // ---------------------------------------------------------------------------
 
 // The far field of precision p (0 FP64, 1 FP32) for the solves `which`
-// selects: source records, then every far-field kernel of that precision
-// (and the FFT solves' block-Toeplitz products) into the partial slices,
-// bracketed by HIP events when timing.
-void Engine::far_field(int p, int which, int in_slot) {
-  hipStream_t st = stream_;
+// selects (of group g, on its stream; g = -1: every solve, on stream_):
+// source records, then every far-field kernel of that precision (and the FFT
+// solves' block-Toeplitz products) into the partial slices, bracketed by HIP
+// events when timing.
+void Engine::far_field(int p, int which, int in_slot, int g) {
+  hipStream_t st = gstream(g);
   const bool fp32 = p == 1;
-  const int nv = (int)vit_.size();
-  const int nfree = (int)far_free_[p].size(), nsub = (int)far_sub_[p].size();
+  const int nv = vcount(g);
+  const VecItem* vi = d_vit_.as<VecItem>() + vlo(g);
+  // group g's range of far list l
+  auto lo = [&](int l) { return g < 0 ? 0 : flo_[p][l][g]; };
+  auto cnt = [&](int l, size_t all) { return g < 0 ? (int)all : flo_[p][l][g + 1] - flo_[p][l][g]; };
+  const int nfree = cnt(0, far_free_[p].size()), nclean = cnt(1, far_clean_[p].size());
+  const int nsub = cnt(2, far_sub_[p].size());
+  const int newt = cnt(3, far_ewt_[p].size()), newp = cnt(4, far_ewp_[p].size());
+  const FarItem* ifree = d_far_free_[p].as<FarItem>() + lo(0);
+  const FarItem* iclean = d_far_clean_[p].as<FarItem>() + lo(1);
+  const FarItem* isub = d_far_sub_[p].as<FarItem>() + lo(2);
+  const FarItem* iewt = d_far_ewt_[p].as<FarItem>() + lo(3);
+  const FarItem* iewp = d_far_ewp_[p].as<FarItem>() + lo(4);
   const bool rot = tiling_.block[p] == kSymRotBlock;
   const bool sym = tiling_.block[p] == kSymBlock || rot;
   const int blk = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
-  const VecItem* vi = d_vit_.as<VecItem>();
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
+  double2* part = d_part_.as<double2>();
   EvPair ev{nullptr, nullptr, fp32};
   if (timing_) {
     LH_HIP(hipEventCreate(&ev.a));
     LH_HIP(hipEventCreate(&ev.b));
     (cap_evs_ ? *cap_evs_ : evs_).push_back(ev);
   }
-  const int newt = (int)far_ewt_[p].size(), newp = (int)far_ewp_[p].size();
   // an FP32 matvec keeps its partial slices in FP32
   const int p32 = part32(fp32);
   if (fp32) {
@@ -683,62 +764,55 @@ void Engine::far_field(int p, int which, int in_slot) {
     if (newp > 0)
       launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                     d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
-    if (timing_) record(ev.a);
+    if (timing_) record(ev.a, st);
     if (sym) {
-      launch_far_sym32(st, true, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
-                       d_src32_.as<Src32>(), d_part_.as<double2>(), p32);
-      launch_far_sym32(st, false, (int)far_clean_[1].size(), d_far_clean_[1].as<FarItem>(), sd,
-                       gs, which, d_src32_.as<Src32>(), d_part_.as<double2>(), p32);
+      launch_far_sym32(st, true, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part, p32);
+      launch_far_sym32(st, false, nclean, iclean, sd, gs, which, d_src32_.as<Src32>(), part, p32);
     } else {
-      launch_far32(st, false, nfree, d_far_free_[1].as<FarItem>(), sd, gs, which,
-                   d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk, p32);
-      launch_far_clean32(st, (int)far_clean_[1].size(), d_far_clean_[1].as<FarItem>(), sd, gs,
-                         which, d_src32_.as<Src32>(), d_part_.as<double2>(),
+      launch_far32(st, false, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part,
+                   tiling_.hw_sincos != 0, blk, p32);
+      launch_far_clean32(st, nclean, iclean, sd, gs, which, d_src32_.as<Src32>(), part,
                          tiling_.hw_sincos != 0, blk, p32);
     }
-    launch_far32(st, true, nsub, d_far_sub_[1].as<FarItem>(), sd, gs, which,
-                 d_src32_.as<Src32>(), d_part_.as<double2>(), tiling_.hw_sincos != 0, blk, p32);
-    launch_ewt_shift32(st, newt, d_far_ewt_[1].as<FarItem>(), sd, gs, which,
-                       d_src32_.as<Src32>(), d_gpack_.as<int>(), d_gizj2_.as<int>(),
-                       d_ewt32_.as<float>(), d_part_.as<double2>(), p32);
+    launch_far32(st, true, nsub, isub, sd, gs, which, d_src32_.as<Src32>(), part,
+                 tiling_.hw_sincos != 0, blk, p32);
+    launch_ewt_shift32(st, newt, iewt, sd, gs, which, d_src32_.as<Src32>(), d_gpack_.as<int>(),
+                       d_gizj2_.as<int>(), d_ewt32_.as<float>(), part, p32);
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
-    if (timing_) record(ev.a);
+    if (timing_) record(ev.a, st);
     if (rot) {
-      launch_far_symrot64(st, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
-                          d_src64_.as<Src64>(), d_part_.as<double2>());
+      launch_far_symrot64(st, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
     } else if (sym) {
-      launch_far_sym64(st, true, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
-                       d_src64_.as<Src64>(), d_part_.as<double2>());
-      launch_far_sym64(st, false, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd,
-                       gs, which, d_src64_.as<Src64>(), d_part_.as<double2>());
+      launch_far_sym64(st, true, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
+      launch_far_sym64(st, false, nclean, iclean, sd, gs, which, d_src64_.as<Src64>(), part);
     } else {
-      launch_far64(st, false, nfree, d_far_free_[0].as<FarItem>(), sd, gs, which,
-                   d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
-      launch_far_clean64(st, (int)far_clean_[0].size(), d_far_clean_[0].as<FarItem>(), sd, gs,
-                         which, d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
+      launch_far64(st, false, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part, blk);
+      launch_far_clean64(st, nclean, iclean, sd, gs, which, d_src64_.as<Src64>(), part, blk);
     }
-    launch_far64(st, true, nsub, d_far_sub_[0].as<FarItem>(), sd, gs, which,
-                 d_src64_.as<Src64>(), d_part_.as<double2>(), blk);
-    launch_ewt_shift64(st, newt, d_far_ewt_[0].as<FarItem>(), sd, gs, which,
-                       d_src64_.as<Src64>(), d_gpack_.as<int>(), d_gizj2_.as<int>(),
-                       d_ewt64_.as<double>(), d_part_.as<double2>());
+    launch_far64(st, true, nsub, isub, sd, gs, which, d_src64_.as<Src64>(), part, blk);
+    launch_ewt_shift64(st, newt, iewt, sd, gs, which, d_src64_.as<Src64>(), d_gpack_.as<int>(),
+                       d_gizj2_.as<int>(), d_ewt64_.as<double>(), part);
   }
-  launch_ewp(st, newp, d_far_ewp_[p].as<FarItem>(), sd, gs, which, d_src64_.as<Src64>(),
-             d_ewl_.as<double>(), d_part_.as<double2>(), blk, p32);
-  fft_matvec(in_slot, fp32);
-  if (timing_) record(ev.b);
+  launch_ewp(st, newp, iewp, sd, gs, which, d_src64_.as<Src64>(), d_ewl_.as<double>(), part, blk,
+             p32);
+  fft_matvec(in_slot, fp32, g, st);
+  if (timing_) record(ev.b, st);
 }
 
-// y = A x for the solves `which` selects.
-void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32) {
-  far_field(fp32 ? 1 : 0, which, in_slot);
-  launch_reduce(stream_, (int)vit_.size(), d_vit_.as<VecItem>(), d_solves_.as<SolveDev>(),
+// y = A x for the solves `which` selects (of group g).
+void Engine::reduce(int which, int in_slot, int out_slot, int mode, bool fp32, int g) {
+  launch_reduce(gstream(g), vcount(g), d_vit_.as<VecItem>() + vlo(g), d_solves_.as<SolveDev>(),
                 d_state_.as<GState>(), which, d_ell_col_.as<int>(), d_ell_tid_.as<int>(),
                 d_tab_.as<double>(), d_ainv_.as<double2>(), d_vec_.as<double2>(), in_slot,
                 out_slot, d_part_.as<double2>(), part32(fp32),
                 fp32 && runs32_ ? d_nslot32_.as<int>() : nullptr, mode, d_norm_.as<double>());
+}
+
+void Engine::matvec(int which, int in_slot, int out_slot, int mode, bool fp32, int g) {
+  far_field(fp32 ? 1 : 0, which, in_slot, g);
+  reduce(which, in_slot, out_slot, mode, fp32, g);
 }
 
 void Engine::matvec_all(int in_slot, bool fp32) {
@@ -747,28 +821,44 @@ void Engine::matvec_all(int in_slot, bool fp32) {
   LH_HIP(hipStreamSynchronize(stream_));
 }
 
-int Engine::read_counter() {
-  int c = 0;
-  LH_HIP(hipMemcpyAsync(&c, d_counter_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
-  LH_HIP(hipStreamSynchronize(stream_));
-  return c;
+// Stream b waits for the work stream a has issued so far.
+void Engine::wait_for(hipStream_t b, hipStream_t a, hipEvent_t e) {
+  LH_HIP(hipEventRecord(e, a));
+  LH_HIP(hipStreamWaitEvent(b, e, 0));
 }
 
-void Engine::arnoldi_step(int j, bool fp32) {
-  hipStream_t st = stream_;
-  const int nv = (int)vit_.size(), S = (int)sd_.size();
-  const VecItem* vi = d_vit_.as<VecItem>();
+// stream_ waits for group 1's stream (a no-op with one group).
+void Engine::join_groups() {
+  if (ng_ == 2) wait_for(stream_, gst_[1], gev_[2]);
+}
+
+// Solves counted by the last check or Arnoldi step, over the groups.
+int Engine::read_counter() {
+  join_groups();
+  int c[2] = {0, 0};
+  LH_HIP(hipMemcpyAsync(c, d_counter_.p, sizeof(int) * ng_, hipMemcpyDeviceToHost, stream_));
+  LH_HIP(hipStreamSynchronize(stream_));
+  return c[0] + c[1];
+}
+
+// Arnoldi step j of group g after its far field: reduction into W,
+// Gram-Schmidt, Givens and the exit test, normalisation of v_{j+1}.
+void Engine::step_rest(int j, bool fp32, int g) {
+  hipStream_t st = gstream(g);
+  const int nv = vcount(g), S = scount(g);
+  const VecItem* vi = d_vit_.as<VecItem>() + vlo(g);
+  const int* sl = slist(g);
   const SolveDev* sd = d_solves_.as<SolveDev>();
   GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
-  matvec(0, j, LH_SLOT_W, 0, fp32);
+  reduce(0, j, LH_SLOT_W, 0, fp32, g);
   if (tiling_.mgs && !fp32) {
     // modified Gram-Schmidt as the reference (iterative.rs:92-96): for
     // i = 0..j, h_ij = <v_i, w> of the current w, then w -= h_ij v_i, one
     // (dot, sum, update) triple of launches per basis vector
     for (int i = 0; i <= j; ++i) {
       launch_dots(st, nv, vi, sd, gs, vec, i, i, d_dot_.as<double2>());
-      launch_fin_dots(st, S, sd, gs, i, i, d_dot_.as<double2>(), 0, d_counter_.as<int>());
+      launch_fin_dots(st, S, sd, gs, i, i, d_dot_.as<double2>(), 0, counter(g), sl);
       launch_update(st, nv, vi, sd, gs, vec, i, i, d_norm_.as<double>());
     }
   } else {
@@ -785,12 +875,17 @@ void Engine::arnoldi_step(int j, bool fp32) {
     for (int pass = 0; pass < passes; ++pass) {
       launch_dots(st, nv, vi, sd, gs, vec, 0, j, d_dot_.as<double2>());
       // pass 0 also clears the counter k_fin_step counts into
-      launch_fin_dots(st, S, sd, gs, 0, j, d_dot_.as<double2>(), pass, d_counter_.as<int>());
+      launch_fin_dots(st, S, sd, gs, 0, j, d_dot_.as<double2>(), pass, counter(g), sl);
       launch_update(st, nv, vi, sd, gs, vec, 0, j, d_norm_.as<double>());
     }
   }
-  launch_fin_step(st, S, sd, gs, j, d_norm_.as<double>(), fp32 ? 1 : 0, d_counter_.as<int>());
+  launch_fin_step(st, S, sd, gs, j, d_norm_.as<double>(), fp32 ? 1 : 0, counter(g), sl);
   launch_scale(st, nv, vi, sd, gs, 2, vec, LH_SLOT_W, -1);
+}
+
+void Engine::arnoldi_step(int j, bool fp32, int g) {
+  far_field(fp32 ? 1 : 0, 0, j, g);
+  step_rest(j, fp32, g);
 }
 
 void Engine::clear_events() {
@@ -806,11 +901,11 @@ void Engine::set_timing(bool on) {
   timing_ = on;
 }
 
-// An event on the engine's stream; during graph capture an event record
-// node of the graph (hipEventRecordExternal), re-recorded at every replay.
-void Engine::record(hipEvent_t e) {
-  if (cap_evs_) LH_HIP(hipEventRecordWithFlags(e, stream_, hipEventRecordExternal));
-  else LH_HIP(hipEventRecord(e, stream_));
+// An event on stream st; during graph capture an event record node of the
+// graph (hipEventRecordExternal), re-recorded at every replay.
+void Engine::record(hipEvent_t e, hipStream_t st) {
+  if (cap_evs_) LH_HIP(hipEventRecordWithFlags(e, st, hipEventRecordExternal));
+  else LH_HIP(hipEventRecord(e, st));
 }
 
 void Engine::harvest(std::vector<EvPair>& evs, bool destroy) {
@@ -849,9 +944,14 @@ Engine::CycleGraph& Engine::cycle_graph(bool mixed, int cycles) {
   LH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
   cap_evs_ = &cg.evs;
   try {
+    // group 1's stream joins the capture, and the capture ends joined
+    if (ng_ == 2) wait_for(gst_[1], st, gev_[2]);
     for (int c = 0; c < cycles; ++c) enqueue_cycle(mixed, false);
+    join_groups();
   } catch (...) {
     cap_evs_ = nullptr;
+    if (ng_ == 2 && hipEventRecord(gev_[2], gst_[1]) == hipSuccess)
+      (void)hipStreamWaitEvent(st, gev_[2], 0);
     (void)hipStreamEndCapture(st, &g);
     if (g) (void)hipGraphDestroy(g);
     for (auto& e : cg.evs) {
@@ -896,8 +996,9 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   launch_copy_slot(st, nv, vi, sdd, vec, -1, LH_SLOT_X);
   launch_copy_slot(st, nv, vi, sdd, vec, LH_SLOT_RHS, LH_SLOT_R);
   launch_vnorm(st, nv, vi, sdd, gs, 3, vec, LH_SLOT_R, d_norm_.as<double>());
-  launch_zero(st, d_counter_.as<int>());
-  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 1, 0, d_counter_.as<int>());
+  launch_zero(st, counter(0));
+  if (ng_ == 2) launch_zero(st, counter(1));
+  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 1, 0, counter(-1), nullptr);
   LH_HIP(hipGetLastError());
   int running = read_counter();
   // Every cycle performs at least one Arnoldi step of every running solve,
@@ -908,6 +1009,9 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   long long cycles = 0;
   int launches = 0;
   const bool graph = use_graph && tiling_.graph;
+  // host-issued cycles: group 1's stream starts after the set-up above (a
+  // captured graph forks it inside)
+  if (ng_ == 2 && !graph) wait_for(gst_[1], st, gev_[2]);
   const int whole = (int)std::min<long long>(kMaxGraphCycles, max_iter / LH_MMAX + 1);
   int next = 1;
   while (running > 0 && cycles <= max_iter + 1) {
@@ -943,6 +1047,10 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
 // solve is left in the loop (one counter read per step); a captured cycle
 // issues all 30.
 void Engine::enqueue_cycle(bool mixed, bool early_exit) {
+  if (ng_ == 2) {
+    enqueue_cycle_groups(mixed, early_exit);
+    return;
+  }
   hipStream_t st = stream_;
   const int nv = (int)vit_.size(), S = (int)sd_.size();
   const VecItem* vi = d_vit_.as<VecItem>();
@@ -951,17 +1059,63 @@ void Engine::enqueue_cycle(bool mixed, bool early_exit) {
   double2* vec = d_vec_.as<double2>();
   launch_scale(st, nv, vi, sdd, gs, 1, vec, LH_SLOT_R, 0);
   for (int j = 0; j < LH_MMAX; ++j) {
-    arnoldi_step(j, mixed);
+    arnoldi_step(j, mixed, -1);
     if (early_exit) {
       LH_HIP(hipGetLastError());
       if (read_counter() == 0) break;
     }
   }
-  launch_backsub(st, S, gs);
+  launch_backsub(st, S, gs, nullptr);
   launch_xupdate(st, nv, vi, sdd, gs, vec);
   matvec(1, LH_SLOT_X, LH_SLOT_R, 1, false);
-  launch_zero(st, d_counter_.as<int>());
-  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, d_counter_.as<int>());
+  launch_zero(st, counter(0));
+  launch_check(st, S, sdd, gs, d_norm_.as<double>(), 0, 1, counter(0), nullptr);
+}
+
+// The same cycle with two solve groups on two streams. The far fields take
+// turns: group 1's far field of step j waits for group 0's, and group 0's
+// far field of step j + 1 for group 1's of step j, so exactly one far field
+// runs at a time, while the other group reduces, orthogonalises and
+// normalises beside it. Each group's kernels keep their order on its stream.
+void Engine::enqueue_cycle_groups(bool mixed, bool early_exit) {
+  const SolveDev* sdd = d_solves_.as<SolveDev>();
+  GState* gs = d_state_.as<GState>();
+  double2* vec = d_vec_.as<double2>();
+  hipStream_t s0 = gst_[0], s1 = gst_[1];
+  for (int g = 0; g < 2; ++g)
+    launch_scale(gstream(g), vcount(g), d_vit_.as<VecItem>() + vlo(g), sdd, gs, 1, vec,
+                 LH_SLOT_R, 0);
+  // the far fields of group 0 then group 1, each group's remaining work after its own
+  // Without the turns (the two groups' chains independent inside the graph)
+  // the batch workload's step measured 21.5-22.0 s against 18.54 s for one
+  // group and 18.06 s with the turns: the far fields then run at once and
+  // slow each other.
+  auto turns = [&](int p, int which, int in_slot, auto&& rest) {
+    far_field(p, which, in_slot, 0);
+    wait_for(s1, s0, gev_[0]);
+    far_field(p, which, in_slot, 1);
+    LH_HIP(hipEventRecord(gev_[1], s1));
+    rest(0);
+    LH_HIP(hipStreamWaitEvent(s0, gev_[1], 0));
+    rest(1);
+  };
+  for (int j = 0; j < LH_MMAX; ++j) {
+    turns(mixed ? 1 : 0, 0, j, [&](int g) { step_rest(j, mixed, g); });
+    if (early_exit) {
+      LH_HIP(hipGetLastError());
+      if (read_counter() == 0) break;
+    }
+  }
+  for (int g = 0; g < 2; ++g) {
+    launch_backsub(gstream(g), scount(g), gs, slist(g));
+    launch_xupdate(gstream(g), vcount(g), d_vit_.as<VecItem>() + vlo(g), sdd, gs, vec);
+  }
+  turns(0, 1, LH_SLOT_X, [&](int g) {
+    reduce(1, LH_SLOT_X, LH_SLOT_R, 1, false, g);
+    launch_zero(gstream(g), counter(g));
+    launch_check(gstream(g), scount(g), sdd, gs, d_norm_.as<double>(), 0, 1, counter(g),
+                 slist(g));
+  });
 }
 
 void Engine::gradients(int p_slot, const double dir[3], double amplitude,
@@ -1170,6 +1324,7 @@ void Engine::fft_kernels() {
     launch_fft_kernel(st, f.len, f.P, f.nn, f.h, d.k, K);
     launch_fft_fcd(st, f.n_fcd, f.fcd_offs.as<int>(), f.len, f.P, f.h, d.cell, d.k, K);
     LH_HIP(hipGetLastError());
+    check_fft(hipfftSetStream(f.plan64, st), "set stream");
     for (int half = 0; half < 2; ++half) {
       hipfftDoubleComplex* k3 = reinterpret_cast<hipfftDoubleComplex*>(K + 3 * half * f.len);
       check_fft(hipfftExecZ2Z(f.plan64, k3, k3, HIPFFT_FORWARD), "exec Z2Z");
@@ -1179,14 +1334,16 @@ void Engine::fft_kernels() {
   }
 }
 
-// G x of every FFT solve into its partial slice 0 (k_reduce subtracts it).
-void Engine::fft_matvec(int in_slot, bool fp32) {
-  hipStream_t st = stream_;
+// G x of every FFT solve (of group g; g = -1: all) into its partial slice 0
+// (k_reduce subtracts it), on stream st.
+void Engine::fft_matvec(int in_slot, bool fp32, int g, hipStream_t st) {
   for (auto& fp : fft_) {
     FftSolve& f = *fp;
+    if (g >= 0 && group_of_[f.s] != g) continue;
     const SolveDev& d = sd_[f.s];
+    check_fft(hipfftSetStream(fp32 ? f.plan32 : f.plan64, st), "set stream");
     const double2* x = d_vec_.as<double2>() + d.vec_off + (long long)in_slot * 3 * d.n;
-    double2* part = d_part_.as<double2>() + d.part_off;
+    double2* part = d_part_.as<double2>();
     const double scale = 1.0 / (double)f.len;
     if (fp32) {
       hipfftComplex* X = f.X32.as<hipfftComplex>();
@@ -1195,14 +1352,15 @@ void Engine::fft_matvec(int in_slot, bool fp32) {
       launch_fft_mul32(st, f.len, f.K32.as<float2>(), f.X32.as<float2>());
       check_fft(hipfftExecC2C(f.plan32, X, X, HIPFFT_BACKWARD), "exec C2C");
       launch_fft_gather32(st, d.n, f.len, f.dip2pt.as<int>(), f.X32.as<float2>(), scale, part,
-                          part32(true));
+                          d.part_off, part32(true));
     } else {
       hipfftDoubleComplex* X = f.X64.as<hipfftDoubleComplex>();
       launch_fft_scatter64(st, f.len, f.pt2dip.as<int>(), x, f.X64.as<double2>());
       check_fft(hipfftExecZ2Z(f.plan64, X, X, HIPFFT_FORWARD), "exec Z2Z");
       launch_fft_mul64(st, f.len, f.K64.as<double2>(), f.X64.as<double2>());
       check_fft(hipfftExecZ2Z(f.plan64, X, X, HIPFFT_BACKWARD), "exec Z2Z");
-      launch_fft_gather64(st, d.n, f.len, f.dip2pt.as<int>(), f.X64.as<double2>(), scale, part);
+      launch_fft_gather64(st, d.n, f.len, f.dip2pt.as<int>(), f.X64.as<double2>(), scale, part,
+                          d.part_off);
     }
   }
 }
@@ -1357,6 +1515,7 @@ std::vector<SolveResult> Engine::results() {
     r[s].matvecs32 = g.matvecs32;
     r[s].residual = (g.rhs_norm > 0.0) ? g.res_norm / g.rhs_norm : 0.0;
     r[s].pair_evals = pairs_per_matvec((int)s) * g.matvecs;
+    r[s].group = group_of_[s];
   }
   return r;
 }

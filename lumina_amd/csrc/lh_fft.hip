@@ -136,17 +136,22 @@ __global__ __launch_bounds__(256) void k_fft_mul(long long len, const T2* __rest
 
 // (G x)_i = X[c][point of i] / len into the far partial slice.
 template <class T2>
+// part is the start of the partial-slice buffer and part_off the solve's
+// slice 0 in complex elements: with p32 the slices are float2 from the
+// buffer's start, so the offset is applied as an element index, as every
+// reader (k_reduce) does.
 __global__ __launch_bounds__(256) void k_fft_gather(int n, long long len,
                                                     const int* __restrict__ dip2pt,
                                                     const T2* __restrict__ X, double scale,
-                                                    double2* __restrict__ part, int p32) {
+                                                    double2* __restrict__ part,
+                                                    long long part_off, int p32) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const long long q = dip2pt[i];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const T2 v = X[c * len + q];
-    put_part(part, 3LL * i + c, (double)v.x * scale, (double)v.y * scale, p32);
+    put_part(part, part_off + 3LL * i + c, (double)v.x * scale, (double)v.y * scale, p32);
   }
 }
 
@@ -192,15 +197,16 @@ void launch_fft_mul32(hipStream_t st, long long len, const float2* K, float2* X)
 }
 
 void launch_fft_gather64(hipStream_t st, int n, long long len, const int* dip2pt,
-                         const double2* X, double scale, double2* part) {
+                         const double2* X, double scale, double2* part, long long part_off) {
   hipLaunchKernelGGL(k_fft_gather<double2>, grid_of(n), dim3(256), 0, st, n, len, dip2pt, X,
-                     scale, part, 0);
+                     scale, part, part_off, 0);
 }
 
 void launch_fft_gather32(hipStream_t st, int n, long long len, const int* dip2pt,
-                         const float2* X, double scale, double2* part, int p32) {
+                         const float2* X, double scale, double2* part, long long part_off,
+                         int p32) {
   hipLaunchKernelGGL(k_fft_gather<float2>, grid_of(n), dim3(256), 0, st, n, len, dip2pt, X,
-                     scale, part, p32);
+                     scale, part, part_off, p32);
 }
 
 }  // namespace lh

@@ -294,12 +294,21 @@ __global__ __launch_bounds__(64) void k_zero(int* __restrict__ counter) {
 // over the items and combine with a fixed butterfly (deterministic).
 // Pass 0 also clears the step counter that k_fin_step of the same Arnoldi
 // step counts into (block 0, before any block leaves).
+//
+// The per-solve kernels below run one block per solve: solve sl[b] for
+// block b when a solve list is given (the solves of one solve group), else
+// solve b.
+__device__ __forceinline__ int solve_of_block(const int* sl) {
+  return sl ? sl[blockIdx.x] : (int)blockIdx.x;
+}
+
 __global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ solves,
                                                  GState* __restrict__ st, int i0, int j,
                                                  const double2* __restrict__ dot_part,
-                                                 int pass, int* __restrict__ counter) {
-  const int s = blockIdx.x;
-  if (pass == 0 && s == 0 && threadIdx.x == 0) *counter = 0;
+                                                 int pass, int* __restrict__ counter,
+                                                 const int* __restrict__ sl) {
+  const int s = solve_of_block(sl);
+  if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
   GState& g = st[s];
   if (!g.inner) return;
   const SolveDev S = solves[s];
@@ -332,8 +341,9 @@ __global__ __launch_bounds__(256) void k_fin_dots(const SolveDev* __restrict__ s
 __global__ __launch_bounds__(64) void k_fin_step(const SolveDev* __restrict__ solves,
                                                 GState* __restrict__ st, int j,
                                                 const double* __restrict__ norm_part,
-                                                int fp32, int* __restrict__ counter) {
-  const int s = blockIdx.x;
+                                                int fp32, int* __restrict__ counter,
+                                                const int* __restrict__ sl) {
+  const int s = solve_of_block(sl);
   GState& g = st[s];
   if (!g.inner) {
     if (threadIdx.x == 0) g.nflag = 0;
@@ -413,8 +423,9 @@ __global__ __launch_bounds__(64) void k_fin_step(const SolveDev* __restrict__ so
 }
 
 // back_substitute (iterative.rs:194-206) for solves in a cycle.
-__global__ __launch_bounds__(64) void k_backsub(GState* __restrict__ st) {
-  const int s = blockIdx.x;
+__global__ __launch_bounds__(64) void k_backsub(GState* __restrict__ st,
+                                               const int* __restrict__ sl) {
+  const int s = solve_of_block(sl);
   GState& g = st[s];
   if (!g.started || threadIdx.x != 0) return;
   const int M = LH_MMAX;
@@ -445,8 +456,9 @@ __global__ __launch_bounds__(64) void k_backsub(GState* __restrict__ st) {
 __global__ __launch_bounds__(64) void k_check(const SolveDev* __restrict__ solves,
                                              GState* __restrict__ st,
                                              const double* __restrict__ norm_part, int init,
-                                             int count_matvec, int* __restrict__ counter) {
-  const int s = blockIdx.x;
+                                             int count_matvec, int* __restrict__ counter,
+                                             const int* __restrict__ sl) {
+  const int s = solve_of_block(sl);
   GState& g = st[s];
   const SolveDev S = solves[s];
   if (!init && !g.started) return;
@@ -537,21 +549,22 @@ void launch_zero(hipStream_t st, int* counter) {
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, counter);
 }
 void launch_fin_dots(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int i0, int j,
-                     const double2* dp, int pass, int* counter) {
+                     const double2* dp, int pass, int* counter, const int* sl) {
   hipLaunchKernelGGL(k_fin_dots, dim3(ns), dim3(256), 0, st, solves, gs, i0, j, dp, pass,
-                     counter);
+                     counter, sl);
 }
 void launch_fin_step(hipStream_t st, int ns, const SolveDev* solves, GState* gs, int j,
-                     const double* np, int fp32, int* counter) {
-  hipLaunchKernelGGL(k_fin_step, dim3(ns), dim3(64), 0, st, solves, gs, j, np, fp32, counter);
+                     const double* np, int fp32, int* counter, const int* sl) {
+  hipLaunchKernelGGL(k_fin_step, dim3(ns), dim3(64), 0, st, solves, gs, j, np, fp32, counter,
+                     sl);
 }
-void launch_backsub(hipStream_t st, int ns, GState* gs) {
-  hipLaunchKernelGGL(k_backsub, dim3(ns), dim3(64), 0, st, gs);
+void launch_backsub(hipStream_t st, int ns, GState* gs, const int* sl) {
+  hipLaunchKernelGGL(k_backsub, dim3(ns), dim3(64), 0, st, gs, sl);
 }
 void launch_check(hipStream_t st, int ns, const SolveDev* solves, GState* gs, const double* np,
-                  int init, int count_matvec, int* counter) {
+                  int init, int count_matvec, int* counter, const int* sl) {
   hipLaunchKernelGGL(k_check, dim3(ns), dim3(64), 0, st, solves, gs, np, init, count_matvec,
-                     counter);
+                     counter, sl);
 }
 
 }  // namespace lh

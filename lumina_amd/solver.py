@@ -498,6 +498,11 @@ class Batch:
     def set_timing(self, on: bool = True):
         check(self._lib.lh_batch_set_timing(self._h, 1 if on else 0))
 
+    def set_graph_cycles(self, cycles: int):
+        """Restart cycles per graph launch of run(): 0 adaptive (1, 2, 4,
+        then 8), N > 0 fixed, -1 the whole solve in one launch."""
+        check(self._lib.lh_batch_set_graph_cycles(self._h, int(cycles)))
+
     def kernel_times(self) -> dict:
         t = _lib.KernelTimes()
         check(self._lib.lh_batch_kernel_times(self._h, C.byref(t)))

@@ -268,3 +268,39 @@ def test_fft_with_offset_dependent_fcd_flags(backend, monkeypatch):
     assert relmax(y_plain, yo) > 1e-6           # the flipped pair matters
     assert relmax(y, yo) < 1e-12
     s.close()
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_fft_solve_behind_other_solves_in_a_batch(backend, monkeypatch, precision):
+    """An FFT solve whose partial slices do not start the buffer (a pair-sum
+    solve before it in the batch) gives the same answer, bit for bit, as the
+    same solve alone in a batch: in mixed precision its FP32 slice is written
+    and read at the same element offset (a mismatch there once read unwritten
+    slices as NaN)."""
+    from lumina_amd.solver import Batch
+    monkeypatch.setenv("LUMINA_FFT", "force")
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", "256")
+    h = 1.5
+    cub = box(6, 5, 4, h=(h, h, h), origin=(-4.0, 2.0, 1.0))
+    k = I.wavenumber(600.0, 1.33)
+    fft = dict(positions=cub, k=k, use_fcd=True, cell_size=h,
+               polarisability=I.polarisability_block("Au", 600.0, h, 1.33))
+    sph = I.au_sphere_r10_d2()
+    other = dict(positions=sph, k=k, use_fcd=True, cell_size=2.0,
+                 polarisability=I.polarisability_block("Au", 600.0, 2.0, 1.33))
+    out = {}
+    for name, probs in (("alone", [fft]), ("behind", [other, fft])):
+        b = Batch(backend, probs)
+        b.run(1e-10, 150, precision)
+        try:
+            cs, infos, p = b.results(want_moments=True)
+        except ConvergenceFailure as e:
+            cs, infos, p = e.cross_sections, e.infos, e.moments
+        b.close()
+        out[name] = (cs[-1], infos[-1], p[-1])
+        print(name, cs[-1], infos[-1])
+    assert out["alone"][1]["pair_evals"] == 0
+    assert np.all(np.isfinite(out["behind"][0]))
+    assert out["behind"][1]["iterations"] == out["alone"][1]["iterations"]
+    assert np.array_equal(out["behind"][0], out["alone"][0])
+    assert np.array_equal(out["behind"][2], out["alone"][2])
