@@ -269,6 +269,7 @@ def run_batch(args):
     kt_tot = {"far32_ms": 0.0, "far32_pairs": 0.0, "far32_launches": 0,
               "far64_ms": 0.0, "far64_pairs": 0.0, "far64_launches": 0}
     trips = []
+    groups = None
     rounds = 0   # far-field launches that had work: lock-step rounds up to the longest solve
     for s_ in range(args.steps):
         t1 = time.perf_counter()
@@ -402,19 +403,23 @@ def run_batch(args):
     if (prec == MIXED and args.particles == 64 and args.wavelengths == 16
             and os.path.exists(tf)):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch_by_solve_groups", {}).get(str(groups),
+                                                                     tj.get("bytes_per_launch"))
     roof = {"bound": "valu", "kernel": kname,
             "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "avg_launch_ms": far_ms / max(rounds, 1), "launches": rounds,
             "event_pairs": far_n,
             "flops_per_pair": FLOPS_PER_FAR_PAIR, "targets_per_item": blk,
             "far_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else 0.0,
-            "events": ("HIP events on the batch stream around every far-field launch of the "
-                       "timed steps, " + ("recorded by the host-issued cycles"
+            "events": ("HIP events on each solve group's stream around every far-field launch "
+                       "of the timed steps (a launch is one group's far field; the groups' far "
+                       "fields take turns and never overlap), " + ("recorded by the host-issued cycles"
                                           if os.environ.get("LUMINA_GRAPH") == "0" else
                                           "event-record nodes of the captured cycle graphs") +
-                       "; avg_launch_ms divides their total by the launches that had work (the "
-                       "lock-step rounds up to the longest solve's matvec count); event_pairs "
+                       "; avg_launch_ms divides their total by the launches that had work (per "
+                       "group, the lock-step rounds up to its longest solve's matvec count); "
+                       "event_pairs "
                        "also counts the launches of cycles issued after every solve had "
                        "finished, whose kernels exit at once")}
 
