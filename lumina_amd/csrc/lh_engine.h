@@ -182,7 +182,7 @@ class Engine {
   void harvest(std::vector<EvPair>& evs, bool destroy);
   // g = -1: every solve on stream_; g = 0, 1: the solves of group g on its
   // stream
-  void far_field(int p, int which, int in_slot, int g = -1);
+  void far_field(int p, int which, int in_slot, int g = -1, hipEvent_t wait_ev = nullptr);
   int part32(bool fp32) const { return (fp32 && tiling_.part32) ? 1 : 0; }
   void matvec(int which, int in_slot, int out_slot, int mode, bool fp32, int g = -1);
   void reduce(int which, int in_slot, int out_slot, int mode, bool fp32, int g);

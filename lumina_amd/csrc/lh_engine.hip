@@ -723,10 +723,10 @@ double Engine::pairs_per_matvec(int s) const {
 
 // The far field of precision p (0 FP64, 1 FP32) for the solves `which`
 // selects (of group g, on its stream; g = -1: every solve, on stream_):
-// source records, then every far-field kernel of that precision (and the FFT
-// solves' block-Toeplitz products) into the partial slices, bracketed by HIP
-// events when timing.
-void Engine::far_field(int p, int which, int in_slot, int g) {
+// source records, then (after wait_ev, when given) every far-field kernel of
+// that precision (and the FFT solves' block-Toeplitz products) into the
+// partial slices, bracketed by HIP events when timing.
+void Engine::far_field(int p, int which, int in_slot, int g, hipEvent_t wait_ev) {
   hipStream_t st = gstream(g);
   const bool fp32 = p == 1;
   const int nv = vcount(g);
@@ -764,6 +764,7 @@ void Engine::far_field(int p, int which, int in_slot, int g) {
     if (newp > 0)
       launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                     d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
+    if (wait_ev) LH_HIP(hipStreamWaitEvent(st, wait_ev, 0));
     if (timing_) record(ev.a, st);
     if (sym) {
       launch_far_sym32(st, true, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part, p32);
@@ -781,6 +782,7 @@ void Engine::far_field(int p, int which, int in_slot, int g) {
   } else {
     launch_prep64(st, nv, vi, sd, gs, which, d_px_.as<double>(), d_py_.as<double>(),
                   d_pz_.as<double>(), vec, in_slot, d_src64_.as<Src64>());
+    if (wait_ev) LH_HIP(hipStreamWaitEvent(st, wait_ev, 0));
     if (timing_) record(ev.a, st);
     if (rot) {
       launch_far_symrot64(st, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
@@ -944,8 +946,12 @@ Engine::CycleGraph& Engine::cycle_graph(bool mixed, int cycles) {
   LH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
   cap_evs_ = &cg.evs;
   try {
-    // group 1's stream joins the capture, and the capture ends joined
-    if (ng_ == 2) wait_for(gst_[1], st, gev_[2]);
+    // group 1's stream joins the capture (gev_[1] then marks its start),
+    // and the capture ends joined
+    if (ng_ == 2) {
+      wait_for(gst_[1], st, gev_[2]);
+      LH_HIP(hipEventRecord(gev_[1], gst_[1]));
+    }
     for (int c = 0; c < cycles; ++c) enqueue_cycle(mixed, false);
     join_groups();
   } catch (...) {
@@ -1011,7 +1017,10 @@ void Engine::solve(double tol, long long max_iter, bool mixed, bool use_graph) {
   const bool graph = use_graph && tiling_.graph;
   // host-issued cycles: group 1's stream starts after the set-up above (a
   // captured graph forks it inside)
-  if (ng_ == 2 && !graph) wait_for(gst_[1], st, gev_[2]);
+  if (ng_ == 2 && !graph) {
+    wait_for(gst_[1], st, gev_[2]);
+    LH_HIP(hipEventRecord(gev_[1], gst_[1]));
+  }
   const int whole = (int)std::min<long long>(kMaxGraphCycles, max_iter / LH_MMAX + 1);
   int next = 1;
   while (running > 0 && cycles <= max_iter + 1) {
@@ -1089,14 +1098,15 @@ void Engine::enqueue_cycle_groups(bool mixed, bool early_exit) {
   // Without the turns (the two groups' chains independent inside the graph)
   // the batch workload's step measured 21.5-22.0 s against 18.54 s for one
   // group and 18.06 s with the turns: the far fields then run at once and
-  // slow each other.
+  // slow each other. Each group's source records are prepared before it
+  // waits for its turn (gev_[1]: group 1's last far field, recorded when
+  // group 1's stream joined; gev_[0]: group 0's far field of this step).
   auto turns = [&](int p, int which, int in_slot, auto&& rest) {
-    far_field(p, which, in_slot, 0);
-    wait_for(s1, s0, gev_[0]);
-    far_field(p, which, in_slot, 1);
+    far_field(p, which, in_slot, 0, gev_[1]);
+    LH_HIP(hipEventRecord(gev_[0], s0));
+    far_field(p, which, in_slot, 1, gev_[0]);
     LH_HIP(hipEventRecord(gev_[1], s1));
     rest(0);
-    LH_HIP(hipStreamWaitEvent(s0, gev_[1], 0));
     rest(1);
   };
   for (int j = 0; j < LH_MMAX; ++j) {
