@@ -1094,13 +1094,14 @@ void Engine::enqueue_cycle_groups(bool mixed, bool early_exit) {
   for (int g = 0; g < 2; ++g)
     launch_scale(gstream(g), vcount(g), d_vit_.as<VecItem>() + vlo(g), sdd, gs, 1, vec,
                  LH_SLOT_R, 0);
-  // the far fields of group 0 then group 1, each group's remaining work after its own
-  // Without the turns (the two groups' chains independent inside the graph)
-  // the batch workload's step measured 21.5-22.0 s against 18.54 s for one
-  // group and 18.06 s with the turns: the far fields then run at once and
-  // slow each other. Each group's source records are prepared before it
-  // waits for its turn (gev_[1]: group 1's last far field, recorded when
-  // group 1's stream joined; gev_[0]: group 0's far field of this step).
+  // The far fields of group 0 then group 1, each group's remaining work
+  // after its own far field. Each group's source records are prepared before
+  // it waits for its turn (gev_[1]: group 1's last far field, first recorded
+  // when group 1's stream joined; gev_[0]: group 0's far field of this
+  // step). Without the turns (the two groups' chains independent inside the
+  // graph) the batch workload's step measured 21.5-22.0 s against 18.54 s
+  // for one group and 18.06 s with the turns: the far fields then run at
+  // once and slow each other.
   auto turns = [&](int p, int which, int in_slot, auto&& rest) {
     far_field(p, which, in_slot, 0, gev_[1]);
     LH_HIP(hipEventRecord(gev_[0], s0));
