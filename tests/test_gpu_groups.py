@@ -149,3 +149,25 @@ def test_graph_cycles_entry_point(backend, monkeypatch):
     for cyc in (-1, 3):
         assert out[cyc][1] == out[0][1]
         assert np.array_equal(out[cyc][0], out[0][0])
+
+
+def test_sweep_groups_bitwise(backend, monkeypatch):
+    """The device wavelength sweep (lh_sweep_solve) solves its wavelengths as
+    one batch: with two solve groups its rows equal the one-group rows bit
+    for bit."""
+    from lumina_amd import sweep as W
+    pos = I.au_sphere_r10_d2()
+    wls = [400.0 + 25.0 * t for t in range(9)]
+    out = {}
+    for g in ("1", "2"):
+        for key in KEYS:
+            monkeypatch.delenv(key, raising=False)
+        monkeypatch.setenv("LUMINA_FAR_BLOCK", "256")
+        monkeypatch.setenv("LUMINA_GROUPS", g)
+        cs, infos = W.run_sweep(backend, pos, wls, [W.johnson_christy("Au")], spacing=2.0,
+                                n_env=1.33, tolerance=1e-10, max_iterations=2000)
+        out[g] = (cs, [i["iterations"] for i in infos], [i["solve_group"] for i in infos])
+        print(g, out[g][1], out[g][2])
+    assert sorted(set(out["2"][2])) == [0, 1] and set(out["1"][2]) == {0}
+    assert out["2"][1] == out["1"][1]
+    assert np.array_equal(out["2"][0], out["1"][0])
