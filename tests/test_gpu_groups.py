@@ -171,3 +171,24 @@ def test_sweep_groups_bitwise(backend, monkeypatch):
     assert sorted(set(out["2"][2])) == [0, 1] and set(out["1"][2]) == {0}
     assert out["2"][1] == out["1"][1]
     assert np.array_equal(out["2"][0], out["1"][0])
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("form", ["256", "sym"])
+def test_solve_independent_of_batch_position(backend, precision, form, monkeypatch):
+    """A solve's result does not depend on the other solves of its batch or
+    on its place among them: every problem of the mixed batch (free-space
+    pair sums, FFT, Fresnel image, 2D lattice) solved alone in a batch of one
+    equals its row of the whole batch bit for bit. Offsets into the shared
+    device buffers (vectors, source records, partial slices, tables) differ
+    between the two runs, so a mix-up of any of them shows here."""
+    probs = mixed_batch()
+    env = {"LUMINA_GROUPS": "2", "LUMINA_FAR_BLOCK": form}
+    cs, infos, p, _ = run(backend, probs, precision, monkeypatch, env)
+    for q, pr in enumerate(probs):
+        cs1, inf1, p1, _ = run(backend, [pr], precision, monkeypatch, env)
+        print(q, infos[q]["iterations"], inf1[0]["iterations"], cs[q], cs1[0])
+        assert np.all(np.isfinite(cs1[0])), q
+        assert inf1[0]["iterations"] == infos[q]["iterations"], q
+        assert np.array_equal(cs1[0], cs[q]), q
+        assert np.array_equal(p1[0], p[q]), q
