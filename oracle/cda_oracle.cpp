@@ -252,8 +252,11 @@ std::vector<double> recip_points(const Lattice2D& L, int ns) {
 }
 
 // ewald.rs:119-174 real_space (2D branch: erfc damping).
-// Parity unpinned beyond invariants (see ewald tests in tests/test_oracle.py):
-// only a restatement pins the reference's Ewald output.
+// Pinned by: the one-shell sum against an independent damped point sum with
+// scipy's erfc times the mpmath-pinned point tensor (to the A&S 7.1.26
+// error), the A&S erfc against scipy, and the invariants of
+// tests/test_oracle.py (split identity, reciprocity, square symmetry, shell
+// convergence).
 void ewald_real(const Lattice2D& L, const double r[3], const double kb[3], double k,
                 T33 g) {
   double rn2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
