@@ -189,6 +189,8 @@ double erfc_real(double x) {
 }
 
 // ewald.rs:334-338 csqrt_positive_imag.
+// Pinned by: numpy's principal square root over every quadrant and both
+// axes, flipped to Im >= 0 (4e-16), and ewald.rs:827-838 known answers.
 C64 csqrt_pos_imag(C64 z) {
   C64 s = csqrt(z);
   return (s.im < 0.0) ? neg(s) : s;
@@ -200,6 +202,9 @@ struct Lattice2D {
   int n_real, n_recip;
 };
 
+// Lattice helpers (lattice.rs:75-233). Pinned by: lattice.rs:243-301 known
+// answers (square lattice b_i = 2 pi / a, eta = sqrt(pi / A)), b_i . a_j =
+// 2 pi delta_ij in the spectral-sum closed forms of tests/test_oracle_ewald.py.
 // lattice.rs:75-89 cell_measure (2D branch).
 double cell_measure(const Lattice2D& L) {
   double cx = L.a1[1] * L.a2[2] - L.a1[2] * L.a2[1];
@@ -285,6 +290,12 @@ void ewald_real(const Lattice2D& L, const double r[3], const double kb[3], doubl
 }
 
 // ewald.rs:186-273 recip_space (2D spectral sum).
+// Pinned by: the closed forms the documented sum reduces to
+// (tests/test_oracle_ewald.py): far above the lattice only the (0,0) order
+// survives, at normal and at oblique incidence; the first evanescent shell
+// of a rectangular lattice; an independent numpy evaluation of the sum; and
+// planted mistakes (prefactor, exponent sign, k_z, phase sign, basis) that
+// each fail one of them.
 void ewald_recip(const Lattice2D& L, const double r[3], const double kb[3], double k,
                  T33 g) {
   double area = cell_measure(L);
@@ -328,6 +339,8 @@ void ewald_eval(const Lattice2D& L, const double r[3], const double kb[3], doubl
 }
 
 // assembly.rs:335-363 invert_3x3 (cofactor inverse).
+// Pinned by: identity, scalar and random complex matrices against
+// numpy.linalg.inv (tests/test_oracle.py).
 void invert3(const C64 m[9], C64 out[9]) {
   C64 a = m[0], b = m[1], cc = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
       k = m[8];
@@ -468,6 +481,8 @@ void direct_tensor_grad(const Problem& P, int64_t i, int64_t j, T333 dg) {
 }
 
 // assembly.rs:209-318 matvec_on_the_fly, one row block i.
+// Pinned by: the dense assembled matrix times x (1e-15, FCD and image terms
+// included) and a plain-Python 4-dipole sum (tests/test_oracle.py).
 void matvec_row(const Problem& P, const C64* x, int64_t i, C64 yi[3]) {
   const double* ri = P.pos + 3 * i;
   C64 inv[9];
@@ -539,6 +554,8 @@ void matvec(const Problem& P, const C64* x, C64* y, int nthreads) {
   for (auto& t : th) t.join();
 }
 
+// Vector kernels and Givens rotations of GMRES (iterative.rs:163-206).
+// Pinned by: the GMRES known systems and LAPACK solutions below.
 // iterative.rs:163-168 vector_dot: sum(conj(a_i) * b_i), sequential.
 C64 vdot(const C64* a, const C64* b, int64_t n) {
   C64 s = c(0.0);
@@ -830,6 +847,8 @@ void or_matvec_rows(const OrSpec* spec, const double* x, double* y, int64_t row_
 
 // assembly.rs:45-175 assemble_interaction_matrix (dense, row-major 3N x 3N);
 // used with LAPACK as an independent check of the on-the-fly matvec + GMRES.
+// Pinned by: a 4-dipole matrix built in plain Python from the golden-checked
+// G (tests/test_oracle.py), and A x equal to the on-the-fly matvec.
 void or_assemble(const OrSpec* spec, double* out) {
   Lattice2D L;
   Problem P = to_problem(spec, &L);
@@ -921,6 +940,8 @@ int or_solve_gmres_dense(const double* Adense, int64_t n, const double* rhs, dou
 }
 
 // types.rs:60-74 IncidentField::at_position for every dipole
+// Pinned by: the numpy plane wave at an oblique direction with a complex
+// polarisation (tests/test_oracle.py).
 // (assembly.rs:178-194 build_incident_field_vector). The polarisation is
 // complex here so that circular polarisation (no reference counterpart)
 // uses the same routine; a real polarisation gives the reference's values.
@@ -941,6 +962,9 @@ void or_incident(int64_t n, const double* pos, const double* dir, const double* 
 }
 
 // cda/mod.rs:187-235 C_ext / C_abs / C_sca.
+// Pinned by: the single dipole's C_ext = k Im(alpha) (cda_diagnostic.rs:
+// 119-167) with C_abs + C_sca = C_ext, and energy balance on a particle
+// (tests/test_oracle.py).
 void or_cross_sections(int64_t n, const double* alpha, int alpha_uniform, const double* p,
                        const double* einc, double k, double amplitude, double* out3) {
   const C64* A = reinterpret_cast<const C64*>(alpha);
@@ -1033,6 +1057,9 @@ int or_adjoint_grad(const OrSpec* spec, const double* dir, const double* pol6, d
 // fields.rs:18-48 field_at_point at m observation points: the incident
 // field (types.rs:60-74) plus sum_i G(r_obs, r_i) p_i, dipoles with
 // |r_obs - r_i|^2 < 1e-20 skipped; also |E|^2 (fields.rs:87).
+// Pinned by: the scattered field at k r = 1e4 against k^2 e^{ikr}/(4 pi r)
+// (I - rhat rhat) F, the incident plane wave alone for zero moments, and the
+// skip of a coincident dipole (tests/test_oracle.py).
 void or_field_at_points(int64_t n, const double* pos, const double* moments, double k,
                         const double* dir, const double* pol6, double amplitude, int64_t m,
                         const double* points, double* field, double* intensity) {
@@ -1060,6 +1087,9 @@ void or_field_at_points(int64_t n, const double* pos, const double* moments, dou
 // fields.rs:156-221 compute_far_field: theta = pi it / max(n_theta - 1, 1),
 // phi = 2 pi ip / n_phi, F = sum_i p_i e^{-ik rhat.r_i}, intensity
 // |(I - rhat rhat) F|^2.
+// Pinned by: one dipole's |(I - rhat rhat) p|^2 (phase-free), two equal
+// dipoles L apart on z giving that times |1 + e^{-ikL cos theta}|^2, and the
+// theta/phi grid (tests/test_oracle.py).
 void or_far_field(int64_t n, const double* pos, const double* moments, double k, int64_t n_theta,
                   int64_t n_phi, double* theta, double* phi, double* intensity) {
   const C64* P = reinterpret_cast<const C64*>(moments);

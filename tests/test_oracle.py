@@ -128,6 +128,21 @@ def test_csqrt_matches_principal_sqrt_everywhere():
     assert worst < 4e-16, worst
 
 
+def test_incident_is_the_plane_wave():
+    """types.rs:60-74 / assembly.rs:178-194: E_i = E0 e exp(i k d.r_i),
+    interleaved [3i + c], against numpy at an oblique direction with a
+    complex (circular) polarisation and a non-unit amplitude."""
+    r = np.random.default_rng(3)
+    pos = r.uniform(-20, 20, size=(7, 3))
+    k = 2 * np.pi * 1.33 / 550.0
+    d = np.array([0.3, -0.4, np.sqrt(1 - 0.25)])
+    pol = np.array([1.0, 1j, 0.0]) / np.sqrt(2.0)
+    P = O.Problem(pos, I.polarisability_block("Au", 550.0, 2.0, 1.33), k, True, 2.0)
+    e = P.incident(direction=tuple(d), polarisation=tuple(pol), amplitude=2.5)
+    want = (2.5 * np.exp(1j * k * (pos @ d)))[:, None] * pol[None, :]
+    assert np.max(np.abs(e.reshape(-1, 3) - want)) < 1e-14
+
+
 def test_lattice_helpers():
     # lattice.rs:243-301
     a1, a2 = [100.0, 0, 0], [0, 100.0, 0]
