@@ -448,6 +448,10 @@ def run_batch(args):
                             f"converge in both precisions)") if fp64_batch else None,
         "fp64_batched": fp64_batch, "fp64_plain": plain, "adaptive_graph_launches": adaptive,
         "speedup_vs_fp64_plain_pair_evals": (value / plain["pair_evals_per_s"]) if plain else None,
+        # solves/s of the whole batch over solves/s of the plain sessions on
+        # their sample (every particle at evenly spaced wavelengths)
+        "speedup_vs_fp64_plain_solves": ((S * args.steps / total) / plain["solves_per_s"])
+                                        if plain else None,
         "roofline": roof,
     }
     if not args.no_cpu_baseline and infos64 is not None:
