@@ -437,6 +437,57 @@ class CdaSolver:
         local = np.einsum("nab,nb->na", ainv, moments)
         return DipoleResponse(wavelength_nm, params.environment_n, moments, local, info)
 
+    def _response_session(self, positions, polarisability, k: float):
+        if len(positions) == 0:
+            raise LuminaError(_lib.LH_INVALID_ARGUMENT, "No dipoles provided")
+        return PairSumSession(self.backend, positions, polarisability, k, self.use_fcd,
+                              self.cell_size, None, FP64)
+
+    def cross_sections_from_response(self, positions, polarisability, response: DipoleResponse,
+                                     params: SimulationParams) -> CrossSections:
+        """cda/mod.rs:123-166: C_ext, C_abs, C_sca of already solved moments
+        (no solve), k from the response's wavelength and params.environment_n,
+        the device reductions of lh_pairsum_cross_sections."""
+        k = wavenumber(response.wavelength_nm, params.environment_n)
+        s = self._response_session(positions, polarisability, k)
+        try:
+            e_inc = s.incident(self.incident_field)
+            out = s.cross_sections(np.asarray(response.moments).reshape(-1), e_inc,
+                                   self.incident_field.amplitude)
+        finally:
+            s.close()
+        return CrossSections(response.wavelength_nm, float(out[0]), float(out[1]), float(out[2]))
+
+    def compute_near_field(self, positions, polarisability, response: DipoleResponse, plane):
+        """cda/mod.rs:421-430 -> fields.rs:51-108 on the device
+        (lh_pairsum_field_at_points): |E|^2 on the plane's grid, k from the
+        response's wavelength and medium, this solver's incident field."""
+        from .fields import compute_near_field_map
+        k = wavenumber(response.wavelength_nm, response.environment_n)
+        s = self._response_session(positions, polarisability, k)
+        try:
+            return compute_near_field_map(s, plane, self.incident_field,
+                                          np.asarray(response.moments).reshape(-1))
+        finally:
+            s.close()
+
+    def compute_far_field(self, positions, polarisability, response: DipoleResponse,
+                          n_theta: int, n_phi: int):
+        """cda/mod.rs:432-441 -> fields.rs:156-221 on the device
+        (lh_pairsum_far_field)."""
+        from .fields import compute_far_field
+        k = wavenumber(response.wavelength_nm, response.environment_n)
+        s = self._response_session(positions, polarisability, k)
+        try:
+            return compute_far_field(s, n_theta, n_phi, np.asarray(response.moments).reshape(-1),
+                                     response.wavelength_nm)
+        finally:
+            s.close()
+
+    def method_name(self) -> str:
+        """cda/mod.rs:443-445."""
+        return "Coupled Dipole Approximation (CDA)"
+
 
 class Batch:
     """A device-resident batch of compute_cross_sections problems

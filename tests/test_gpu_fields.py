@@ -192,3 +192,44 @@ def test_full_size_near_field_tends_to_far_field(backend):
     ff2 = F.compute_far_field(s, 3, 4, p)
     assert relmax(ff2.intensity, io) < 1e-12
     s.close()
+
+
+def test_cda_solver_response_methods(backend):
+    """The rest of the reference's OpticalSolver surface on CdaSolver
+    (cda/mod.rs:123-166, 421-445): cross sections from an already solved
+    response equal those of the solve and the oracle's formula on the same
+    moments; the near-field map and the far-field pattern of the response
+    equal the oracle's fields.rs restatement; the method name."""
+    from lumina_amd.solver import CdaSolver, SimulationParams
+    lam, n_env, d = 540.0, 1.33, 2.0
+    pos = I.au_sphere_r10_d2()
+    alpha = I.polarisability_block("Au", lam, d, n_env)
+    inc = IncidentField(direction=(0.0, 0.0, 1.0), polarisation=(1.0, 0.0, 0.0), amplitude=1.5)
+    solver = CdaSolver(backend, incident_field=inc, use_fcd=True, cell_size=d)
+    params = SimulationParams(environment_n=n_env, solver_tolerance=1e-12, max_iterations=2000)
+    assert solver.method_name() == "Coupled Dipole Approximation (CDA)"
+    resp = solver.solve_dipoles(pos, alpha, lam, params)
+    cs = solver.compute_cross_sections(pos, alpha, lam, params)
+    got = solver.cross_sections_from_response(pos, alpha, resp, params)
+    a = np.array([got.extinction, got.absorption, got.scattering])
+    b = np.array([cs.extinction, cs.absorption, cs.scattering])
+    k = I.wavenumber(lam, n_env)
+    P = O.Problem(pos, alpha, k, True, d)
+    p = resp.moments.reshape(-1)
+    want = P.cross_sections(p, P.incident(amplitude=1.5), 1.5)
+    print("from response", a, "solve", b, "oracle", want)
+    assert got.wavelength_nm == lam
+    assert np.all(np.abs(a - b) <= 1e-10 * abs(b[0]))
+    assert np.all(np.abs(a - want) <= 1e-12 * abs(want[0]))
+    plane = F.NearFieldPlane(centre=(0.0, 0.0, 0.0), normal=(0.0, 1.0, 1.0), half_width=25.0,
+                             half_height=15.0, nx=13, ny=7)
+    nf = solver.compute_near_field(pos, alpha, resp, plane)
+    pts, _ = F.plane_points(plane)
+    _, io = O.field_at_points(pos, p, k, pts, amplitude=1.5)
+    assert nf.nx == 13 and nf.ny == 7 and np.array_equal(nf.positions, pts)
+    assert relmax(nf.field_intensity, io) < 1e-12
+    ff = solver.compute_far_field(pos, alpha, resp, 7, 8)
+    th, ph, fo = O.far_field(pos, p, k, 7, 8)
+    assert ff.wavelength_nm == lam and ff.n_theta == 7 and ff.n_phi == 8
+    assert np.array_equal(ff.theta, th) and np.array_equal(ff.phi, ph)
+    assert relmax(ff.intensity, fo) < 1e-12
