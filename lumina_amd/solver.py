@@ -353,6 +353,39 @@ class HipBackend:
     def create_session(self, n: int) -> MatvecSession:
         return MatvecSession(self, n)
 
+    def parallel_matrix_fill(self, rows: int, cols: int, fill_fn) -> np.ndarray:
+        """ComputeBackend::parallel_matrix_fill (backend.rs:78-83): entry
+        (i, j) = fill_fn(i, j), row-major. A host-side fill of a caller's
+        closure, as the reference's GPU backend does it (gpu.rs:611-632)."""
+        out = np.empty((int(rows), int(cols)), dtype=np.complex128)
+        for i in range(int(rows)):
+            for j in range(int(cols)):
+                out[i, j] = fill_fn(i, j)
+        return out
+
+    def matvec(self, matrix, vector) -> np.ndarray:
+        """ComputeBackend::matvec as the GPU backend gives it (gpu.rs:634-655):
+        a one-off product through a temporary stored-matrix session. A
+        non-square matrix or a vector of another length is LH_DEVICE_ERROR
+        (ComputeError::DeviceError), as there."""
+        m = _c128(matrix)
+        v = _c128(vector).reshape(-1)
+        if m.ndim != 2 or m.shape[0] != m.shape[1] or m.shape[0] != v.size:
+            shape = "x".join(str(d) for d in m.shape)
+            raise LuminaError(_lib.LH_DEVICE_ERROR,
+                              f"Dimension mismatch: matrix {shape}, vector {v.size}")
+        s = MatvecSession(self, m.shape[0])
+        try:
+            s.upload_matrix(m)
+            return s.matvec(v)
+        finally:
+            s.close()
+
+    def dense_solve(self, matrix, rhs):
+        """ComputeBackend::dense_solve (backend.rs:119-134): the GPU backends
+        keep the trait's default, ComputeError::Unavailable."""
+        raise LuminaError(_lib.LH_UNAVAILABLE, "Dense solve not implemented for this backend")
+
     def create_pairsum_session(self, *a, **kw) -> PairSumSession:
         return PairSumSession(self, *a, **kw)
 

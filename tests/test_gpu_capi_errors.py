@@ -109,6 +109,28 @@ def test_stored_matrix_session_errors(backend):
     ms.close()
 
 
+def test_backend_trait_methods(backend):
+    """The rest of ComputeBackend (backend.rs:62-134) as the reference's GPU
+    backend gives it: the one-off matvec through a temporary session
+    (gpu.rs:634-655) against numpy, with its dimension-mismatch DeviceError;
+    the host-side matrix fill (gpu.rs:611-632); dense_solve Unavailable."""
+    r = np.random.default_rng(5)
+    n = 37
+    A = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    x = r.standard_normal(n) + 1j * r.standard_normal(n)
+    y = backend.matvec(A, x)
+    assert np.max(np.abs(y - A @ x)) < 1e-13 * np.max(np.abs(A @ x))
+    st, msg = status_of(backend.matvec, A, x[:-1])
+    assert st == _lib.LH_DEVICE_ERROR and "Dimension mismatch" in msg
+    st, msg = status_of(backend.matvec, A[:, :-1], x[:-1])
+    assert st == _lib.LH_DEVICE_ERROR
+    F = backend.parallel_matrix_fill(3, 4, lambda i, j: complex(i, 10 * j))
+    assert F.shape == (3, 4) and F[2, 3] == 2 + 30j and F[1, 0] == 1
+    st, msg = status_of(backend.dense_solve, A, x)
+    assert st == _lib.LH_UNAVAILABLE
+    assert np.allclose(backend.matvec(np.eye(n), x), x, rtol=0, atol=0)
+
+
 def test_batch_errors(backend):
     st, msg = status_of(Batch, backend, [])
     assert st == _lib.LH_INVALID_ARGUMENT
