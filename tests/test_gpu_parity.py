@@ -74,6 +74,10 @@ def test_matvec_substrate_and_anisotropic(backend):
     x = rand_x(len(pos), 2)
     yo = O.Problem(pos, blocks, k, True, 2.0, substrate=(-1.0, sub.delta_eps)).matvec(x)
     assert rel(s.matvec(x, FP64), yo) < 1e-12
+    # the FP32 far field with the image term (k_far_row<float, true, ...>)
+    e32 = rel(s.matvec(x, MIXED), yo)
+    print(f"substrate + anisotropic fp32 matvec rel err {e32:.2e}")
+    assert e32 < 2e-6
 
 
 def I_delta():
@@ -89,7 +93,13 @@ def test_matvec_random_positions(backend):
     k = 2 * np.pi / 500.0
     s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.7)
     x = rand_x(300, 4)
-    assert rel(s.matvec(x, FP64), O.Problem(pos, alpha, k, True, 1.7).matvec(x)) < 1e-12
+    yo = O.Problem(pos, alpha, k, True, 1.7).matvec(x)
+    assert rel(s.matvec(x, FP64), yo) < 1e-12
+    # FP32 far field: the exclusion test on FP32-rounded positions gives the
+    # same pair split as the FP64 near list
+    e32 = rel(s.matvec(x, MIXED), yo)
+    print(f"random positions fp32 matvec rel err {e32:.2e}")
+    assert e32 < 2e-6
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 257])
