@@ -18,7 +18,9 @@ baseline (the oracle's FP64 restatement of the reference CPU path on this
 host's cores).
 
 Other workloads: --workload cuboid (80x40x40 nm Au cuboid, 1.5 nm spacing,
-38,637 dipoles, 520 nm, one solve FP64 and mixed); --workload metasurface
+38,637 dipoles, 520 nm, one solve FP64 and mixed); --workload substrate (the
+same cuboid on glass with the retarded Fresnel image term, one solve FP64 and
+mixed, the image-pair kernel timed by HIP events); --workload metasurface
 (Au cylinder D=100 nm H=40 nm at 2 nm spacing, 41,181 dipoles, on glass with
 the retarded Fresnel image, 300 nm square lattice with 2D Ewald sums, Bloch
 k = 0, a batch of wavelengths in 500..890 nm, FP64 and mixed); --workload
@@ -519,6 +521,75 @@ def run_cuboid(args):
     }
 
 
+def run_substrate(args):
+    """The free-space Fresnel image term (assembly.rs:278-295): the 80x40x40 nm
+    Au cuboid (38,637 dipoles, d = 1.5 nm) at 520 nm in water on glass (eps
+    2.25, retarded factor, interface half a spacing below the lowest dipole
+    plane). Per matvec N(N-1) direct pairs plus N^2 image pairs. One FP64 and
+    one mixed solve, each a batch of one whose far-field launches (direct and
+    image tensor per pair, k_far_row<R, true, ...>) are timed by HIP events."""
+    from lumina_amd import inputs as I
+    from lumina_amd.solver import (FP64, MIXED, Batch, HipBackend, SubstrateRuntime,
+                                   substrate_reflection_factor)
+    pos = I.au_cuboid_80x40x40_d1p5()
+    lam, n_env, d = 520.0, 1.33, 1.5
+    k = I.wavenumber(lam, n_env)
+    sub = SubstrateRuntime(z_interface=float(pos[:, 2].min()) - 0.5 * d,
+                           delta_eps=substrate_reflection_factor(2.25 + 0j, n_env, True))
+    prob = dict(positions=pos, k=k, use_fcd=True, cell_size=d, substrate=sub,
+                polarisability=I.polarisability_block("Au", lam, d, n_env))
+    n = len(pos)
+    be = HipBackend(0)
+    res = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        b = Batch(be, [prob])
+        b.set_timing(True)
+        for _ in range(args.warmup):
+            b.run(args.tol, args.max_iter, prec)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run(args.tol, args.max_iter, prec)
+        dt = (time.perf_counter() - t1) / args.steps
+        cs, infos = results_allow_failures(b)
+        kt = b.kernel_times()
+        b.close()
+        info = infos[0]
+        far_ms = kt["far32_ms"] if prec == MIXED else kt["far64_ms"]
+        far_pairs = kt["far32_pairs"] if prec == MIXED else kt["far64_pairs"]
+        far_n = kt["far32_launches"] if prec == MIXED else kt["far64_launches"]
+        # direct pairs 90 flops, image pairs 91: 90.5 per pair of the count
+        tflops = 90.5 * far_pairs / (far_ms * 1e-3) / 1e12 if far_ms > 0 else None
+        peak = FP32_PEAK_TFLOPS if prec == MIXED else FP64_PEAK_TFLOPS
+        res[name] = {"s": dt, "pair_evals_per_s": info["pair_evals"] / dt, "cs": cs[0].tolist(),
+                     "info": info, "far_kernel_ms_per_launch": far_ms / far_n if far_n else None,
+                     "far_kernel_pairs_per_s": far_pairs / (far_ms * 1e-3) if far_ms > 0 else None,
+                     "far_kernel_tflops": tflops,
+                     "far_kernel_frac": tflops / peak if tflops else None,
+                     "far_block": kt["far32_block"] if prec == MIXED else kt["far64_block"]}
+        log(f"[bench] substrate {name}: {dt:.4f} s, {info['iterations']} iterations, far kernel "
+            f"{res[name]['far_kernel_pairs_per_s']:.3e} pairs/s ({tflops:.1f} TFLOP/s)")
+    err = abs(res["mixed"]["cs"][0] - res["fp64"]["cs"][0]) / abs(res["fp64"]["cs"][0])
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as O
+        P = O.Problem(pos, prob["polarisability"], k, True, d,
+                      substrate=(sub.z_interface, sub.delta_eps))
+        cpu = cpu_matvec_rate(P, n, pairs(n) + float(n) * n, rows=2048)
+        log(f"[bench] substrate cpu baseline {cpu['value']:.3e} pair-evals/s")
+    return {
+        "cpu_baseline": cpu,
+        "metric": "Green pair-evals/s (single mixed-precision CDA solve on a substrate)",
+        "value": res["mixed"]["pair_evals_per_s"], "unit": "pair-evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * res["mixed"]["s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 far field + fp64 Krylov", "data": "synthetic",
+        "config": {"workload": f"Au cuboid 80x40x40 nm, d=1.5 nm ({n} dipoles), 520 nm, water, "
+                               f"on glass (retarded Fresnel image), FCD, GMRES(30) tol "
+                               f"{args.tol:g}", "parallelism": "single GPU"},
+        "fp64": res["fp64"], "mixed": res["mixed"], "cext_rel_err_mixed_vs_fp64": err,
+    }
+
+
 def metasurface_problems(n_wl, lam_lo=500.0, lam_hi=890.0, n_env=1.0):
     """Au cylinder D=100 H=40 at d=2 (41,181 dipoles) on glass (eps 2.25,
     retarded Fresnel factor, interface 1 nm below the lowest dipole plane:
@@ -954,8 +1025,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "cuboid", "metasurface", "sphere", "helix",
-                                           "fields", "sweep", "copy"],
+    ap.add_argument("--workload", choices=["batch", "cuboid", "substrate", "metasurface", "sphere",
+                                           "helix", "fields", "sweep", "copy"],
                     default="batch")
     ap.add_argument("--fd-dipoles", type=int, default=8)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
@@ -974,7 +1045,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding this path "
               f"over several GPUs is designed but not built.", file=sys.stderr)
         sys.exit(2)
-    run = {"batch": run_batch, "cuboid": run_cuboid, "metasurface": run_metasurface,
+    run = {"batch": run_batch, "cuboid": run_cuboid, "substrate": run_substrate,
+           "metasurface": run_metasurface,
            "sphere": run_sphere, "helix": run_helix, "fields": run_fields, "sweep": run_sweep, "copy": run_copy}
     line = run[args.workload](args)
     print(json.dumps(line), flush=True)
