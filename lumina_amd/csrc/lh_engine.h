@@ -208,6 +208,8 @@ class Engine {
   int vlo_[3] = {0, 0, 0};          // group g's vector items: vit_[vlo_[g] .. vlo_[g + 1])
   int flo_[2][5][3] = {};           // far lists [precision][free, clean, sub, ewt, ewp]:
                                     // group g's items [flo_[p][l][g], flo_[p][l][g + 1])
+  int subn_[2][3] = {};             // one-tensor-per-pair form: group g's near items lead
+                                    // its range of the substrate list
   DevMem d_order_;
   hipEvent_t gev_[3] = {nullptr, nullptr, nullptr};   // far field of group 0 / 1 done; join
   hipStream_t gstream(int g) const { return g <= 0 ? stream_ : gst_[1]; }

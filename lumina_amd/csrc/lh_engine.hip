@@ -26,10 +26,10 @@ void launch_far_clean64(hipStream_t, int, const FarItem*, const SolveDev*, const
                         const Src64*, double2*, int);
 void launch_far_clean32(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                         const Src32*, double2*, bool, int, int);
-void launch_far_sym32(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                      const Src32*, double2*, int);
-void launch_far_sym64(hipStream_t, bool, int, const FarItem*, const SolveDev*, const GState*, int,
-                      const Src64*, double2*);
+void launch_far_sym32(hipStream_t, bool, bool, int, const FarItem*, const SolveDev*, const GState*,
+                      int, const Src32*, double2*, int);
+void launch_far_sym64(hipStream_t, bool, bool, int, const FarItem*, const SolveDev*, const GState*,
+                      int, const Src64*, double2*);
 void launch_far_symrot64(hipStream_t, int, const FarItem*, const SolveDev*, const GState*, int,
                          const Src64*, double2*);
 void launch_reduce(hipStream_t, int, const VecItem*, const SolveDev*, const GState*, int,
@@ -464,6 +464,23 @@ void Engine::build_far_items(int p) {
   const int tb = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
   // FP32 one-tensor-per-pair form: items are runs of source tiles
   const bool runs = p == 1 && sym && !rot;
+  // free-space solves on a substrate in the one-tensor-per-pair form (the
+  // LDS-staged kernels, not the rotating one): their items T >= S go to
+  // far_sub_, within each group the near items (subn_) before the clean ones
+  const bool subsym = sym && !rot;
+  std::vector<FarItem> subn, subc;
+  auto flush_sub = [&](int g) {
+    if (runs)
+      for (auto* v : {&subn, &subc})
+        std::stable_sort(v->begin(), v->end(), [](const FarItem& a, const FarItem& b) {
+          return a.c1 - a.c0 > b.c1 - b.c0;
+        });
+    subn_[p][g] = (int)subn.size();
+    far_sub_[p].insert(far_sub_[p].end(), subn.begin(), subn.end());
+    far_sub_[p].insert(far_sub_[p].end(), subc.begin(), subc.end());
+    subn.clear();
+    subc.clear();
+  };
   std::vector<int>& nslot = nslot32_;
   if (runs) {
     nslot.assign(vit_.size(), 0);
@@ -476,12 +493,15 @@ void Engine::build_far_items(int p) {
   // solves in group order: each group's items are one range of every list
   for (int oi = 0; oi < (int)order_.size(); ++oi) {
     for (int g = 0; g < ng_; ++g)
-      if (oi == glo_[g])
+      if (oi == glo_[g]) {
+        if (g > 0) flush_sub(g - 1);
         for (int l = 0; l < 5; ++l) flo_[p][l][g] = (int)lists[l]->size();
+      }
     const size_t s = (size_t)order_[oi];
     const SolveDev& d = sd_[s];
     if (d.fft) continue;   // G x by FFT
     const bool free_space = d.lat == 0 && !d.has_sub;
+    const bool sub_pairs = subsym && d.lat == 0 && d.has_sub;
     // tile pairs whose boxes lie beyond the exclusion radius (with a margin
     // for FP32 rounding of |d|^2) have no excluded pair and no i == j;
     // decided per 256 x 256 tile pair, so that every block size sends the
@@ -496,7 +516,7 @@ void Engine::build_far_items(int p) {
           box_gap2(box64_[probs_[s].geom], ta / 64, (tz + 63) / 64, ca / 64, (cz + 63) / 64);
       return gap2 > lim && gap2 > 0.0;
     };
-    if (runs && free_space) {
+    if (runs && (free_space || sub_pairs)) {
       // target tile T: runs of up to run_max consecutive source tiles S <= T
       // of one class; slots of T's dipoles: column sums from the tiles
       // T' > T (slot T' - T - 1), then one row-sum slot per run
@@ -508,7 +528,10 @@ void Engine::build_far_items(int p) {
           while (tk <= ti && tk - tj < run_max && clean(ti, tk) == c) ++tk;
           const FarItem f{(int)s, ti * LH_TILE, tj * LH_TILE, std::min(d.n, tk * LH_TILE),
                           d.nt - 1 - ti + r};
-          (c ? far_clean_[p] : far_free_[p]).push_back(f);
+          if (sub_pairs)
+            (c ? subc : subn).push_back(f);
+          else
+            (c ? far_clean_[p] : far_free_[p]).push_back(f);
           tj = tk;
         }
         nslot[(size_t)d.vi_off + ti] = d.nt - 1 - ti + r;
@@ -522,13 +545,18 @@ void Engine::build_far_items(int p) {
         const FarItem f{(int)s, ti * tbs, tj * LH_TILE, std::min(d.n, (tj + 1) * LH_TILE), tj};
         if (d.lat == 1) far_ewt_[p].push_back(f);
         else if (d.lat == 2) far_ewp_[p].push_back(f);
-        else if (d.has_sub) far_sub_[p].push_back(f);
+        else if (sub_pairs) {
+          if (ti >= tj) (clean(ti, tj) ? subc : subn).push_back(f);
+        } else if (d.has_sub) {
+          far_sub_[p].push_back(f);
+        }
         else if (!(sym && free_space) || (ti >= tj && !rot))
           (clean(f.t0 / LH_TILE, tj) ? far_clean_[p] : far_free_[p]).push_back(f);
         else if (ti >= tj)
           far_free_[p].push_back(f);
       }
   }
+  flush_sub(ng_ - 1);
   for (int l = 0; l < 5; ++l)
     for (int g = ng_; g < 3; ++g) flo_[p][l][g] = (int)lists[l]->size();
   if (runs) {
@@ -745,6 +773,17 @@ void Engine::far_field(int p, int which, int in_slot, int g, hipEvent_t wait_ev)
   const bool rot = tiling_.block[p] == kSymRotBlock;
   const bool sym = tiling_.block[p] == kSymBlock || rot;
   const int blk = (tiling_.block[p] > 0 && !sym) ? tiling_.block[p] : LH_TILE;
+  // substrate solves in the one-tensor-per-pair form: per group, the near
+  // items of far_sub_ and then the clean ones (fn(near, count, items))
+  const bool subsym = sym && !rot;
+  auto sub_pair_launches = [&](auto&& fn) {
+    for (int gg = g < 0 ? 0 : g; gg <= (g < 0 ? ng_ - 1 : g); ++gg) {
+      const int a = flo_[p][2][gg], m = subn_[p][gg], e = flo_[p][2][gg + 1];
+      const FarItem* base = d_far_sub_[p].as<FarItem>();
+      fn(true, m, base + a);
+      fn(false, e - a - m, base + a + m);
+    }
+  };
   const SolveDev* sd = d_solves_.as<SolveDev>();
   const GState* gs = d_state_.as<GState>();
   double2* vec = d_vec_.as<double2>();
@@ -767,16 +806,23 @@ void Engine::far_field(int p, int which, int in_slot, int g, hipEvent_t wait_ev)
     if (wait_ev) LH_HIP(hipStreamWaitEvent(st, wait_ev, 0));
     if (timing_) record(ev.a, st);
     if (sym) {
-      launch_far_sym32(st, true, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part, p32);
-      launch_far_sym32(st, false, nclean, iclean, sd, gs, which, d_src32_.as<Src32>(), part, p32);
+      launch_far_sym32(st, true, false, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part,
+                       p32);
+      launch_far_sym32(st, false, false, nclean, iclean, sd, gs, which, d_src32_.as<Src32>(),
+                       part, p32);
     } else {
       launch_far32(st, false, nfree, ifree, sd, gs, which, d_src32_.as<Src32>(), part,
                    tiling_.hw_sincos != 0, blk, p32);
       launch_far_clean32(st, nclean, iclean, sd, gs, which, d_src32_.as<Src32>(), part,
                          tiling_.hw_sincos != 0, blk, p32);
     }
-    launch_far32(st, true, nsub, isub, sd, gs, which, d_src32_.as<Src32>(), part,
-                 tiling_.hw_sincos != 0, blk, p32);
+    if (subsym)
+      sub_pair_launches([&](bool near, int n, const FarItem* f) {
+        launch_far_sym32(st, near, true, n, f, sd, gs, which, d_src32_.as<Src32>(), part, p32);
+      });
+    else
+      launch_far32(st, true, nsub, isub, sd, gs, which, d_src32_.as<Src32>(), part,
+                   tiling_.hw_sincos != 0, blk, p32);
     launch_ewt_shift32(st, newt, iewt, sd, gs, which, d_src32_.as<Src32>(), d_gpack_.as<int>(),
                        d_gizj2_.as<int>(), d_ewt32_.as<float>(), part, p32);
   } else {
@@ -787,13 +833,19 @@ void Engine::far_field(int p, int which, int in_slot, int g, hipEvent_t wait_ev)
     if (rot) {
       launch_far_symrot64(st, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
     } else if (sym) {
-      launch_far_sym64(st, true, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
-      launch_far_sym64(st, false, nclean, iclean, sd, gs, which, d_src64_.as<Src64>(), part);
+      launch_far_sym64(st, true, false, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part);
+      launch_far_sym64(st, false, false, nclean, iclean, sd, gs, which, d_src64_.as<Src64>(),
+                       part);
     } else {
       launch_far64(st, false, nfree, ifree, sd, gs, which, d_src64_.as<Src64>(), part, blk);
       launch_far_clean64(st, nclean, iclean, sd, gs, which, d_src64_.as<Src64>(), part, blk);
     }
-    launch_far64(st, true, nsub, isub, sd, gs, which, d_src64_.as<Src64>(), part, blk);
+    if (subsym)
+      sub_pair_launches([&](bool near, int n, const FarItem* f) {
+        launch_far_sym64(st, near, true, n, f, sd, gs, which, d_src64_.as<Src64>(), part);
+      });
+    else
+      launch_far64(st, true, nsub, isub, sd, gs, which, d_src64_.as<Src64>(), part, blk);
     launch_ewt_shift64(st, newt, iewt, sd, gs, which, d_src64_.as<Src64>(), d_gpack_.as<int>(),
                        d_gizj2_.as<int>(), d_ewt64_.as<double>(), part);
   }

@@ -16,10 +16,12 @@
 //    tensor per unordered pair (k_far_sym32 in FP32, sources staged in LDS;
 //    k_far_sym64 in FP64) also write the column sums of their sources to the
 //    slice of their target tile (k_far_sym32: to per-tile slots, with one
-//    row-sum slot per run of source tiles). With a substrate (k_far_row<R, true>,
-//    assembly.rs:272-295) the image tensor at the mirrored source is applied
+//    row-sum slot per run of source tiles). With a substrate
+//    (assembly.rs:272-295) the image tensor at the mirrored source is applied
 //    to (-D, -D, +D) o x_j, for every pair including j == i, skipped when
-//    |d|^2 < 1e-20.
+//    |d|^2 < 1e-20: in the row form by k_far_row<R, true>, and in the
+//    one-tensor-per-pair form by k_far_sym32/64<NEAR, true>, where one image
+//    tensor serves both directions of a pair.
 //  * near field (k_near_tensors + k_reduce): pairs inside the exclusion
 //    radius use FP64 tensors precomputed once per solve for each distinct
 //    displacement of the geometry: the FCD 27-node cell average
@@ -354,6 +356,13 @@ __global__ __launch_bounds__(256) void k_far_row(const FarItem* __restrict__ ite
 // LDS and wave b adds them in wave order, in FP64: results are deterministic.
 // NEAR = false: tile pairs whose boxes lie beyond the exclusion radius (no
 // excluded pair, no coincident pair, never the diagonal), |d|^2 by FMA.
+// SUB = true (free-space solves on a substrate): each pair also carries the
+// Fresnel image term. The image of source j sits at (x_j, y_j, 2 z0 - z_j), so
+// the image displacement of (i, j) is (dx, dy, ez) with ez = z_i + z_j - 2 z0,
+// and that of (j, i) is (-dx, -dy, ez): one image tensor (A, B of the common
+// length) gives y_i += G(dx, dy, ez) w_j and y_j += G(-dx, -dy, ez) w_i, with
+// w = (-D, -D, +D) o x the image-scaled moment of the source record. Image
+// pairs are never excluded; the diagonal adds the self-image (j == i) one way.
 // ---------------------------------------------------------------------------
 
 // a + (the value of lane l + 1; lane 63 takes lane 0's): DPP wave_rol:1 on the
@@ -365,7 +374,7 @@ __device__ __forceinline__ float rol1_add(float a, float t) {
   return r + t;
 }
 
-template <bool NEAR>
+template <bool NEAR, bool SUB = false>
 __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ items,
                                                    const SolveDev* __restrict__ solves,
                                                    const GState* __restrict__ st, int which,
@@ -374,6 +383,8 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
   __shared__ float4 s_a[128];        // x, y, z, v0 of source (e mod 64) of the sub-block
   __shared__ float4 s_b[128];        // v1 .. v4
   __shared__ float s_c[128];         // v5
+  __shared__ float4 s_d[SUB ? 128 : 1];   // SUB: w0 .. w3 (image-scaled x_j)
+  __shared__ float2 s_e[SUB ? 128 : 1];   // SUB: w4, w5
   __shared__ float s_t[4][64][6];    // [wave][source lane][component]: transposed sums
   // [hi, lo][component][thread]: float-float row sums, each thread's own
   // entries, touched once per 32 steps (registers stay free for occupancy)
@@ -390,12 +401,16 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
   const Src32* __restrict__ sp = src + S.src_off;
   const Src32& me = sp[tvalid ? i : (live ? it.t0 + 64 * w : it.t0)];
   const float xi = me.x, yi = me.y, zi = me.z;
-  float vi[6];
+  float vi[6], wi[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0f;
+  for (int q = 0; q < 6; ++q) {
+    vi[q] = tvalid ? me.v[q] : 0.0f;
+    wi[q] = (SUB && tvalid) ? me.w[q] : 0.0f;
+  }
   const float k = (float)S.k, inv_k = (float)S.inv_k;
   const float krev = (float)(S.k * 0.15915494309189535);
   const float rskip2f = S.rskip2f;
+  const float z2 = SUB ? (float)S.z2 : 0.0f;
   const double c0 = S.c0;
   const long long n3 = 3LL * S.n;
   const int tt = it.t0 / LH_TILE;
@@ -422,6 +437,20 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
 #pragma unroll
     for (int q = 0; q < 6; ++q) run[q] = 0.0f;
   };
+  // SUB: the image pair of this lane's target i and source j (position
+  // offset dx, dy, source height zs, image-scaled moment ws of x_j): the
+  // image of j lies at (x_j, y_j, 2 z0 - z_j), so the displacement is
+  // (dx, dy, ez) with ez = z_i + z_j - 2 z0 in both directions. Forward
+  // y_i += G(dx, dy, ez) w_j into run; with t, the transposed contribution
+  // y_j += G(-dx, -dy, ez) w_i onto t (the same A and B).
+  auto image = [&](float dx, float dy, float zs, const float* ws, float* t) {
+    const float ez = zi - (z2 - zs);
+    const float e2 = fmaf(ez, ez, fmaf(dy, dy, dx * dx));
+    float Er, Ei, Fr, Fi;
+    coeffs32<true>(e2, e2 < 1e-20f, k, krev, inv_k, 1.0f, Er, Ei, Fr, Fi);
+    apply<float>(dx, dy, ez, Er, Ei, Fr, Fi, ws, run);
+    if (t) apply<float>(-dx, -dy, ez, Er, Ei, Fr, Fi, wi, t);
+  };
 
   for (int sc = it.c0; sc < it.c1; sc += LH_TILE) {
     const bool diag = NEAR && sc == it.t0;
@@ -438,6 +467,10 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
         s_a[tid] = make_float4(r.x, r.y, r.z, sv ? r.v[0] : 0.0f);
         s_b[tid] = sv ? make_float4(r.v[1], r.v[2], r.v[3], r.v[4]) : make_float4(0, 0, 0, 0);
         s_c[tid] = sv ? r.v[5] : 0.0f;
+        if constexpr (SUB) {
+          s_d[tid] = sv ? make_float4(r.w[0], r.w[1], r.w[2], r.w[3]) : make_float4(0, 0, 0, 0);
+          s_e[tid] = sv ? make_float2(r.w[4], r.w[5]) : make_float2(0, 0);
+        }
       }
       __syncthreads();
       float ta[6] = {0, 0, 0, 0, 0, 0};
@@ -448,11 +481,23 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
       auto steps = [&](int s0, int s1) {
         float4 ca = s_a[lane + s0], cb = s_b[lane + s0];
         float cc = s_c[lane + s0];
+        float4 cd;
+        float2 ce;
+        if constexpr (SUB) {
+          cd = s_d[lane + s0];
+          ce = s_e[lane + s0];
+        }
 #pragma unroll 4
         for (int s = s0; s < s1; ++s) {
           // entry lane + s + 1 <= 127
           const float4 na = s_a[lane + s + 1], nb = s_b[lane + s + 1];
           const float nc = s_c[lane + s + 1];
+          float4 nd;
+          float2 ne;
+          if constexpr (SUB) {
+            nd = s_d[lane + s + 1];
+            ne = s_e[lane + s + 1];
+          }
           const float vs[6] = {ca.w, cb.x, cb.y, cb.z, cb.w, cc};
           const float dx = xi - ca.x, dy = yi - ca.y, dz = zi - ca.z;
           float Ar, Ai, Br, Bi;
@@ -466,6 +511,12 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
           apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
           float t6[6];
           apply_set<float>(dx, dy, dz, Ar, Ai, Br, Bi, vi, t6);
+          if constexpr (SUB) {
+            const float ws[6] = {cd.x, cd.y, cd.z, cd.w, ce.x, ce.y};
+            image(dx, dy, ca.z, ws, t6);
+            cd = nd;
+            ce = ne;
+          }
 #pragma unroll
           for (int q = 0; q < 6; ++q) ta[q] = rol1_add(ta[q], t6[q]);
           ca = na;
@@ -489,7 +540,15 @@ __global__ __launch_bounds__(256) void k_far_sym32(const FarItem* __restrict__ i
             float Ar, Ai, Br, Bi;
             coeffs32<true>(d2f, d2f <= rskip2f, k, krev, inv_k, 1.0f, Ar, Ai, Br, Bi);
             apply<float>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
+            if constexpr (SUB) {
+              const float4 D4 = s_d[e];
+              const float2 E2 = s_e[e];
+              const float ws[6] = {D4.x, D4.y, D4.z, D4.w, E2.x, E2.y};
+              image(dx, dy, A4.z, ws, nullptr);
+            }
           }
+          // SUB: the self-image, j == i (step 0, one way)
+          if constexpr (SUB) image(0.0f, 0.0f, zi, wi, nullptr);
           fold();
           home = 31;
         } else {
@@ -740,6 +799,7 @@ __global__ __launch_bounds__(256) void k_far_sym(const FarItem* __restrict__ ite
 // boxes lie beyond the exclusion radius. Sums run in FP64; the prefactor
 // k^2 / 4 pi is applied once per output. After each sub-block the waves'
 // transposed sums meet in LDS and the owning wave adds them in wave order.
+// SUB = true adds the Fresnel image term as in k_far_sym32<NEAR, true>.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double wave_rol1(double v) {
@@ -750,7 +810,7 @@ __device__ __forceinline__ double wave_rol1(double v) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-template <bool NEAR>
+template <bool NEAR, bool SUB = false>
 __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ items,
                                                    const SolveDev* __restrict__ solves,
                                                    const GState* __restrict__ st, int which,
@@ -761,7 +821,10 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
   __shared__ double2 s_b[128];       // z, v0
   __shared__ double2 s_c[128];       // v1, v2
   __shared__ double2 s_d[128];       // v3, v4
-  __shared__ double2 s_e[128];       // v5, unused
+  __shared__ double2 s_e[128];       // v5, unused (SUB: w0)
+  __shared__ double2 s_f[SUB ? 128 : 1];   // SUB: w1, w2 (image-scaled x_j)
+  __shared__ double2 s_g[SUB ? 128 : 1];   // SUB: w3, w4
+  __shared__ double s_h[SUB ? 128 : 1];    // SUB: w5
   __shared__ double s_t[4][64][6];   // [wave][source lane][component]: transposed sums
   const FarItem it = items[blockIdx.x];
   if (!takes_part(st, it.s, which)) return;
@@ -776,10 +839,13 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
   const Src64& me = sp[tvalid ? i : (live ? it.t0 + 64 * w : it.t0)];
   const double xi = me.x, yi = me.y, zi = me.z;
   const float xif = me.xf, yif = me.yf, zif = me.zf;
-  double vi[6];
+  double vi[6], wi[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) vi[q] = tvalid ? me.v[q] : 0.0;
-  const double k = S.k, inv_k = S.inv_k;
+  for (int q = 0; q < 6; ++q) {
+    vi[q] = tvalid ? me.v[q] : 0.0;
+    wi[q] = (SUB && tvalid) ? me.w[q] : 0.0;
+  }
+  const double k = S.k, inv_k = S.inv_k, z2 = S.z2;
   const float rskip2f = S.rskip2f;
   const bool diag = NEAR && it.t0 == it.c0;
   const double c0 = S.c0;
@@ -798,15 +864,36 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
       s_b[tid] = make_double2(r.z, sv ? r.v[0] : 0.0);
       s_c[tid] = sv ? make_double2(r.v[1], r.v[2]) : make_double2(0.0, 0.0);
       s_d[tid] = sv ? make_double2(r.v[3], r.v[4]) : make_double2(0.0, 0.0);
-      s_e[tid] = make_double2(sv ? r.v[5] : 0.0, 0.0);
+      s_e[tid] = make_double2(sv ? r.v[5] : 0.0, (SUB && sv) ? r.w[0] : 0.0);
+      if constexpr (SUB) {
+        s_f[tid] = sv ? make_double2(r.w[1], r.w[2]) : make_double2(0.0, 0.0);
+        s_g[tid] = sv ? make_double2(r.w[3], r.w[4]) : make_double2(0.0, 0.0);
+        s_h[tid] = sv ? r.w[5] : 0.0;
+      }
     }
     __syncthreads();
     double ta[6] = {0, 0, 0, 0, 0, 0};
     int home = 0;   // lane l ends holding the transposed sum of source (l + home) mod 64
 
-    // one unordered pair: target of this lane, the source held in a..e;
-    // both = false adds the forward direction only
-    auto pair = [&](double2 a, double2 bb, double2 c, double2 d, double2 e, bool both) {
+    // SUB: the image pair of this lane's target i and source j (position
+    // offset dx, dy, source height zs, image-scaled moment ws of x_j): the
+    // image of j lies at (x_j, y_j, 2 z0 - z_j), so the displacement is
+    // (dx, dy, ez) with ez = z_i + z_j - 2 z0 in both directions. Forward
+    // y_i += G(dx, dy, ez) w_j; with both, the transposed contribution
+    // y_j += G(-dx, -dy, ez) w_i (the same A and B).
+    auto image = [&](double dx, double dy, double zs, const double* ws, bool both) {
+      const double ez = zi - (z2 - zs);
+      const double e2 = dx * dx + dy * dy + ez * ez;
+      double Er, Ei, Fr, Fi;
+      g_coeffs64(e2, k, inv_k, 1.0, Er, Ei, Fr, Fi, e2 < 1e-20);
+      apply<double>(dx, dy, ez, Er, Ei, Fr, Fi, ws, run);
+      if (both) apply<double>(-dx, -dy, ez, Er, Ei, Fr, Fi, wi, ta);
+    };
+    // one unordered pair: target of this lane, the source held in a..e (and
+    // with SUB its image-scaled moment in e.y, f, g, h); both = false adds
+    // the forward direction only
+    auto pair = [&](double2 a, double2 bb, double2 c, double2 d, double2 e, double2 f, double2 g,
+                    double h, bool both) {
       const double vs[6] = {bb.y, c.x, c.y, d.x, d.y, e.x};
       const double dx = xi - a.x, dy = yi - a.y, dz = zi - bb.x;
       double Ar, Ai, Br, Bi;
@@ -818,6 +905,10 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
       }
       apply<double>(dx, dy, dz, Ar, Ai, Br, Bi, vs, run);
       if (both) apply<double>(dx, dy, dz, Ar, Ai, Br, Bi, vi, ta);
+      if constexpr (SUB) {
+        const double ws[6] = {e.y, f.x, f.y, g.x, g.y, h};
+        image(dx, dy, bb.x, ws, both);
+      }
     };
     // steps s0 .. s1 - 1, both directions; the source of step s + 1 is read
     // while step s computes
@@ -829,9 +920,19 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
         // entry lane + s + 1 <= 127
         const int e1 = lane + s + 1;
         const double2 na = s_a[e1], nb = s_b[e1], nc = s_c[e1], nd = s_d[e1], ne = s_e[e1];
+        // SUB: the rest of the image-scaled moment is read in its own step
+        // (its loads overlap the direct pair), not one step ahead: fewer
+        // registers held across the steps
+        double2 cf{}, cg{};
+        double ch = 0.0;
+        if constexpr (SUB) {
+          cf = s_f[lane + s];
+          cg = s_g[lane + s];
+          ch = s_h[lane + s];
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) ta[q] = wave_rol1(ta[q]);
-        pair(ca, cb, cc, cd, ce, true);
+        pair(ca, cb, cc, cd, ce, cf, cg, ch, true);
         ca = na;
         cb = nb;
         cc = nc;
@@ -847,7 +948,12 @@ __global__ __launch_bounds__(256) void k_far_sym64(const FarItem* __restrict__ i
         // each lane adds its own direction only
         steps(1, 32);
         const int e = lane + 32;
-        pair(s_a[e], s_b[e], s_c[e], s_d[e], s_e[e], false);
+        if constexpr (SUB)
+          pair(s_a[e], s_b[e], s_c[e], s_d[e], s_e[e], s_f[e], s_g[e], s_h[e], false);
+        else
+          pair(s_a[e], s_b[e], s_c[e], s_d[e], s_e[e], double2{}, double2{}, 0.0, false);
+        // SUB: the self-image, j == i (step 0, one way)
+        if constexpr (SUB) image(0.0, 0.0, zi, wi, false);
         home = 31;
       } else {
         steps(0, 64);
@@ -1072,17 +1178,26 @@ void launch_far_clean32(hipStream_t st, int nitems, const FarItem* items, const 
                        items, solves, gs, which, src, part, p32);
 }
 
-// near = false: items without excluded pairs (k_far_sym32<false>).
-void launch_far_sym32(hipStream_t st, bool near, int nitems, const FarItem* items,
+// near = false: items without excluded pairs (k_far_sym32<false>); sub:
+// solves on a substrate (the image term, k_far_sym32<., true>).
+void launch_far_sym32(hipStream_t st, bool near, bool sub, int nitems, const FarItem* items,
                       const SolveDev* solves, const GState* gs, int which, const Src32* src,
                       double2* part, int p32) {
   if (nitems <= 0) return;
-  if (near)
+  if (sub) {
+    if (near)
+      hipLaunchKernelGGL((k_far_sym32<true, true>), dim3(nitems), dim3(256), 0, st, items, solves,
+                         gs, which, src, part, p32);
+    else
+      hipLaunchKernelGGL((k_far_sym32<false, true>), dim3(nitems), dim3(256), 0, st, items,
+                         solves, gs, which, src, part, p32);
+  } else if (near) {
     hipLaunchKernelGGL((k_far_sym32<true>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
                        which, src, part, p32);
-  else
+  } else {
     hipLaunchKernelGGL((k_far_sym32<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
                        which, src, part, p32);
+  }
 }
 
 // The rotating form (k_far_sym<double>), every item T >= S with the
@@ -1094,17 +1209,26 @@ void launch_far_symrot64(hipStream_t st, int nitems, const FarItem* items, const
                      which, src, part);
 }
 
-// near = false: items without excluded pairs (k_far_sym64<false>).
-void launch_far_sym64(hipStream_t st, bool near, int nitems, const FarItem* items,
+// near = false: items without excluded pairs (k_far_sym64<false>); sub:
+// solves on a substrate (the image term, k_far_sym64<., true>).
+void launch_far_sym64(hipStream_t st, bool near, bool sub, int nitems, const FarItem* items,
                       const SolveDev* solves, const GState* gs, int which, const Src64* src,
                       double2* part) {
   if (nitems <= 0) return;
-  if (near)
+  if (sub) {
+    if (near)
+      hipLaunchKernelGGL((k_far_sym64<true, true>), dim3(nitems), dim3(256), 0, st, items, solves,
+                         gs, which, src, part);
+    else
+      hipLaunchKernelGGL((k_far_sym64<false, true>), dim3(nitems), dim3(256), 0, st, items,
+                         solves, gs, which, src, part);
+  } else if (near) {
     hipLaunchKernelGGL((k_far_sym64<true>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
                        which, src, part);
-  else
+  } else {
     hipLaunchKernelGGL((k_far_sym64<false>), dim3(nitems), dim3(256), 0, st, items, solves, gs,
                        which, src, part);
+  }
 }
 
 void launch_reduce(hipStream_t st, int nitems, const VecItem* items, const SolveDev* solves,

@@ -85,6 +85,68 @@ def I_delta():
     return substrate_reflection_factor(2.25 + 0.0j, 1.0, True)
 
 
+@pytest.mark.parametrize("n", [1, 2, 65, 257, 515])
+@pytest.mark.parametrize("form", ["sym", "256"])
+def test_matvec_substrate_pair_forms(backend, n, form, monkeypatch):
+    """The Fresnel image term (assembly.rs:278-295) in the one-tensor-per-
+    unordered-pair kernels (form sym: k_far_sym64/32<., true>, where each
+    image tensor serves both directions and the diagonal adds the self-image
+    j == i one way) and in the row kernels (256), FP64 and FP32, against the
+    oracle: a single dipole (the self-image alone), ragged tiles and the
+    515-dipole sphere on glass."""
+    monkeypatch.setenv("LUMINA_FAR_BLOCK", form)
+    if n == 515:
+        pos, alpha, k, cell = sphere_problem()
+    else:
+        r = np.random.default_rng(200 + n)
+        pos = np.unique(np.round(r.uniform(-12, 12, size=(n, 3)) / 2.0) * 2.0, axis=0)
+        alpha = I.isotropic_block(40.0 + 10.0j)
+        k = 2 * np.pi / 480.0
+        cell = 2.0
+    z0 = float(pos[:, 2].min()) - 1.5
+    sub = SubstrateRuntime(z_interface=z0, delta_eps=I_delta())
+    s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=cell, substrate=sub)
+    x = rand_x(len(pos), 31)
+    yo = O.Problem(pos, alpha, k, True, cell, substrate=(z0, sub.delta_eps)).matvec(x)
+    e64, e32 = rel(s.matvec(x, FP64), yo), rel(s.matvec(x, MIXED), yo)
+    print(f"substrate form {form} n={len(pos)}: fp64 {e64:.2e}, fp32 {e32:.2e}")
+    assert e64 < 1e-12
+    assert e32 < 2e-6
+
+
+@pytest.mark.parametrize("runs", ["1", "2", "5"])
+def test_substrate_pair_form_clean_tiles_and_solve(backend, runs, monkeypatch):
+    """A rod of 97 x 4 x 4 dipoles on glass, ordered along its axis: tile
+    pairs beyond the exclusion radius take the image kernels without the
+    exclusion test, in runs of 1, 2 and 5 source tiles (FP32). Matvecs in
+    both precisions against the oracle, and solves in the pair form against
+    the row form (tol 1e-10)."""
+    g = np.stack(np.meshgrid(np.arange(97), np.arange(4), np.arange(4), indexing="ij"), -1)
+    pos = 1.5 * g.reshape(-1, 3).astype(float)
+    alpha = I.isotropic_block(40.0 + 10.0j)
+    k = 2 * np.pi / 480.0
+    z0 = -1.0
+    sub = SubstrateRuntime(z_interface=z0, delta_eps=I_delta())
+    x = rand_x(len(pos), 32)
+    yo = O.Problem(pos, alpha, k, True, 1.5, substrate=(z0, sub.delta_eps)).matvec(x)
+    monkeypatch.setenv("LUMINA_RUN_TILES", runs)
+    out = {}
+    for form in ("sym", "256"):
+        monkeypatch.setenv("LUMINA_FAR_BLOCK", form)
+        s = backend.create_pairsum_session(pos, alpha, k, use_fcd=True, cell_size=1.5,
+                                           substrate=sub)
+        e64, e32 = rel(s.matvec(x, FP64), yo), rel(s.matvec(x, MIXED), yo)
+        cs, _, info = s.compute_cross_sections(IncidentField(), 1e-10, 2000)
+        s.close()
+        print(f"rod on glass, form {form}, runs {runs}: fp64 {e64:.2e}, fp32 {e32:.2e}, "
+              f"{info['iterations']} iterations, C_ext {cs[0]:.10e}")
+        assert e64 < 1e-12
+        assert e32 < 2e-6
+        out[form] = (cs, info["iterations"])
+    assert abs(out["sym"][1] - out["256"][1]) <= 1
+    assert np.all(np.abs(out["sym"][0] - out["256"][0]) <= 1e-9 * abs(out["256"][0][0]))
+
+
 def test_matvec_random_positions(backend):
     # off-grid positions, no coincident pairs, FCD radius crossing many pairs
     r = np.random.default_rng(3)
